@@ -1,0 +1,179 @@
+/*
+ * vtx -- MI355X-native XPBD cloth solver, C ABI.
+ *
+ * The drop-in boundary for the reference's GPU cloth path. Every entry point below names the
+ * reference interface it replaces (file:line in vitalight/Velvet). Conventions:
+ *
+ *  - Handle based, no globals: a vtx_solver owns one HIP stream and all its device buffers.
+ *    Calls on different handles may run from different threads; calls on one handle may not.
+ *  - Every function returns 0 on success or a negative VTX_E* code; vtx_last_error() gives
+ *    the message (per calling thread). Nothing calls exit().
+ *  - Vectors cross the boundary as packed xyz float triples on the host.
+ *  - A solver holds one or more *segments*. A segment is one independent simulation with its
+ *    own parameters, particles, constraints, colliders and spatial-hash grid -- exactly the
+ *    state of one reference VtClothSolverGPU. Segment 0 exists from creation, so a caller that
+ *    mirrors the reference uses segment 0 only. More segments are packed into the same
+ *    launches (batching); they never interact.
+ *  - Particle and slot indices are local to their segment.
+ */
+#ifndef VTX_H
+#define VTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VTX_OK 0
+#define VTX_EINVAL (-1)     /* bad argument */
+#define VTX_EHIP (-2)       /* HIP runtime error */
+#define VTX_ENOMEM (-3)     /* device or host allocation failed */
+#define VTX_ESTATE (-4)     /* call not valid in the current state */
+#define VTX_ECAPACITY (-5)  /* exceeds a stated maximum */
+
+#define VTX_COLLIDER_SPHERE 0   /* Velvet/Common.hpp:113-118 (ColliderType order) */
+#define VTX_COLLIDER_PLANE 1
+#define VTX_COLLIDER_CUBE 2
+
+/* Solver parameters. Replaces VtSimParams (Velvet/Common.hpp:19-48); defaults from
+ * vtx_params_default() are the reference's. num_substeps, num_iterations, interleaved_hash and
+ * max_num_neighbors must agree across the segments of one solver. */
+typedef struct vtx_params {
+    int32_t num_substeps;            /* 2 */
+    int32_t num_iterations;          /* 4 */
+    int32_t max_num_neighbors;       /* 64 (at most 64) */
+    float max_speed;                 /* 50; add_cloth sets 2 d / (1/60) * num_substeps */
+    float gravity[3];                /* (0, -9.8, 0) */
+    float bend_compliance;           /* 0 */
+    float damping;                   /* 0.25 */
+    float relaxation_factor;         /* 1 (SOR omega of the averaged apply) */
+    float long_range_stretchiness;   /* 1.2 */
+    float collision_margin;          /* 0.06 */
+    float friction;                  /* 0.1 */
+    int32_t enable_self_collision;   /* 1 */
+    int32_t interleaved_hash;        /* 3: rebuild neighbours every n substeps */
+    float particle_diameter;         /* set by add_cloth (last cloth wins) */
+    float particle_diameter_scalar;  /* 1.5 (used by mesh-side generators) */
+    float hash_cell_size_scalar;     /* 1.5: cell = diameter * scalar */
+    int32_t hash_table_size;         /* extension: 0 = 2 x particles (reference) */
+    float stretch_stiffness;         /* extension: lambda multiplier; 1 = reference */
+} vtx_params;
+
+/* Collider. Replaces the host Collider -> SDFCollider packing
+ * (Velvet/VtClothSolverGPU.hpp:187-205, Velvet/VtClothSolverGPU.cuh:8-18). Matrices are 4x4
+ * column-major (glm layout): cur is this frame's transform, last the previous frame's; a
+ * static collider passes last == cur. delta_time is the frame time the motion spans. */
+typedef struct vtx_collider {
+    int32_t type;
+    float position[3];
+    float scale[3];
+    float cur[16];
+    float last[16];
+    float delta_time;
+} vtx_collider;
+
+typedef struct vtx_solver vtx_solver;
+
+void vtx_params_default(vtx_params* p);
+const char* vtx_last_error(void);
+const char* vtx_version(void);
+
+/* Create / destroy. Replaces VtClothSolverGPU construction and Start()
+ * (Velvet/VtClothSolverGPU.hpp:27-33) and OnDestroy (hpp:50-54). */
+int vtx_solver_create(const vtx_params* params, vtx_solver** out);
+int vtx_solver_destroy(vtx_solver* s);
+
+/* Parameters of a segment. Replaces writes to Global::simParams (Velvet/Global.hpp:24). */
+int vtx_set_params(vtx_solver* s, int32_t seg, const vtx_params* params);
+int vtx_get_params(vtx_solver* s, int32_t seg, vtx_params* out);
+
+/* Batching extension: add an independent segment; returns its index. */
+int vtx_add_segment(vtx_solver* s, const vtx_params* params, int32_t* seg_out);
+int vtx_num_segments(vtx_solver* s, int32_t* out);
+
+/* Replaces int AddCloth(shared_ptr<Mesh>, glm::mat4, float) (Velvet/VtClothSolverGPU.hpp:114-156)
+ * including InitializePositions (Velvet/VtClothSolverGPU.cu:30-40): positions are transformed
+ * by model (column-major 4x4, may be NULL for identity), inverse masses start at 1, triangle
+ * indices are offset, the segment's diameter and max speed are set, and the collision-filter
+ * snapshot of initial positions is retaken for the whole segment. *offset receives the index
+ * of the cloth's first particle. */
+int vtx_add_cloth(vtx_solver* s, int32_t seg, const float* xyz, uint32_t n, const uint32_t* tris,
+                  uint32_t ntri, const float* model, float diameter, int32_t* offset);
+
+/* Replaces AddStretch (Velvet/VtClothSolverGPU.hpp:158-163). ij holds 2*count indices. */
+int vtx_add_stretch(vtx_solver* s, int32_t seg, int32_t i, int32_t j, float rest);
+int vtx_add_stretch_n(vtx_solver* s, int32_t seg, const int32_t* ij, const float* rest, uint32_t count);
+
+/* Replaces AddAttachSlot (Velvet/VtClothSolverGPU.hpp:165-168); *slot receives its index. */
+int vtx_add_attach_slot(vtx_solver* s, int32_t seg, const float pos[3], int32_t* slot);
+/* Replaces host writes to attachSlotPositions (e.g. Velvet/main.cpp:326-331). */
+int vtx_set_attach_slot_position(vtx_solver* s, int32_t seg, int32_t slot, const float pos[3]);
+
+/* Replaces AddAttach (Velvet/VtClothSolverGPU.hpp:170-176): distance 0 pins the particle. */
+int vtx_add_attach(vtx_solver* s, int32_t seg, int32_t particle, int32_t slot, float dist);
+int vtx_add_attach_n(vtx_solver* s, int32_t seg, const int32_t* particle, const int32_t* slot,
+                     const float* dist, uint32_t count);
+
+/* Replaces AddBend (Velvet/VtClothSolverGPU.hpp:178-185). idx holds 4*count indices. */
+int vtx_add_bend(vtx_solver* s, int32_t seg, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3,
+                 float angle);
+int vtx_add_bend_n(vtx_solver* s, int32_t seg, const uint32_t* idx, const float* angle, uint32_t count);
+
+/* Replaces UpdateColliders (Velvet/VtClothSolverGPU.hpp:187-205). Only enabled colliders are
+ * passed (the reference keeps stale slots for disabled ones). */
+int vtx_set_colliders(vtx_solver* s, int32_t seg, const vtx_collider* cols, uint32_t n);
+
+/* Replaces Simulate() (Velvet/VtClothSolverGPU.hpp:56-111): one frame of every segment.
+ * Asynchronous on the solver's stream; vtx_synchronize waits for it. */
+int vtx_simulate(vtx_solver* s, float frame_dt);
+int vtx_synchronize(vtx_solver* s);
+
+/* Launch-mode switches. use_graph: record each frame's launches into a hipGraph and replay
+ * it (default 1). */
+int vtx_set_use_graph(vtx_solver* s, int32_t use_graph);
+/* Bind the solver to an existing stream (hipStream_t), or NULL for the solver's own. */
+int vtx_set_stream(vtx_solver* s, void* hip_stream);
+
+/* Replaces the public device buffers positions / normals / velocities / invMasses
+ * (Velvet/VtClothSolverGPU.hpp:207-231): D2H reads of a segment's particles as xyz. */
+int vtx_num_particles(vtx_solver* s, int32_t seg, uint32_t* n);
+int vtx_read_positions(vtx_solver* s, int32_t seg, float* xyz);
+int vtx_read_velocities(vtx_solver* s, int32_t seg, float* xyz);
+int vtx_read_normals(vtx_solver* s, int32_t seg, float* xyz);
+int vtx_read_inv_masses(vtx_solver* s, int32_t seg, float* w);
+/* Zero-copy view of device positions / normals: float4 per particle (xyz, w = inverse mass
+ * for positions), *ptr points at the segment's first particle. Valid until the next
+ * registration call. Replaces the CUDA-GL interop mirror (Velvet/VtBuffer.hpp:188-236). */
+int vtx_device_positions(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
+int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
+
+/* Host edit of one particle, the MouseGrabber effect (Velvet/MouseGrabber.hpp:51-52,60,74-75).
+ * pos/vel may be NULL to keep them; inv_mass < 0 keeps it. */
+int vtx_set_particle(vtx_solver* s, int32_t seg, int32_t i, const float* pos, const float* vel,
+                     float inv_mass);
+
+/* Spatial-hash stage on explicit positions, for inspection. Replaces
+ * SpatialHashGPU::Hash (Velvet/SpatialHashGPU.hpp:41-52) / HashObjects
+ * (Velvet/SpatialHashGPU.cu:159-196). vtx_write_predicted sets a segment's predicted
+ * positions; vtx_run_hash hashes every segment's predicted positions; vtx_read_hash returns
+ * the segment's table (cell_start / cell_end, M entries), its sorted hashes and particle
+ * indices (n each) and its neighbour table (max_num_neighbors x n, column-major, entries
+ * after a 0xffffffff sentinel are undefined). Any output pointer may be NULL. */
+int vtx_write_predicted(vtx_solver* s, int32_t seg, const float* xyz);
+int vtx_run_hash(vtx_solver* s);
+int vtx_hash_table_size(vtx_solver* s, int32_t seg, uint32_t* M);
+int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* cell_end,
+                  uint32_t* sorted_hash, uint32_t* sorted_index, uint32_t* nbr);
+
+/* Per-stage GPU time of the last frames, in ms, accumulated since the last reset, by the
+ * reference's timer labels (Velvet/GUI.cpp:32-51). Only filled when timing is on; timing
+ * turns the graph off. */
+int vtx_set_timing(vtx_solver* s, int32_t on);
+int vtx_read_timing(vtx_solver* s, char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+* hash stage: cell tables, sorted order and neighbour lists bit-exact against the binary32
+  restatement, for single and packed segments, with duplicates and caps;
+* trajectories: within 1e-5 x extent of the FP64 oracle after one substep and within
+  1e-3 x extent after 60 frames (config 1);
+* closed-form cases through the whole pipeline;
+* packing invariance: a segment gives bit-identical results alone or packed with others,
+  graph or no graph, run after run.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from velvet_amd import scenes as S
+from velvet_amd.solver import ClothSolver
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+SENT = 0xFFFFFFFF
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _check_hash_equal(gpu, ref, n):
+    cs, ce, sh, si, nb = gpu
+    ph, rsh, rsi, rcs, rce, rnb = ref
+    np.testing.assert_array_equal(cs, rcs)
+    valid = rcs != SENT
+    np.testing.assert_array_equal(ce[valid], rce[valid])
+    np.testing.assert_array_equal(sh, rsh)
+    np.testing.assert_array_equal(si, rsi)
+    assert oracle.neighbor_lists(nb) == oracle.neighbor_lists(rnb)
+
+
+def _folded(n=16, seed=0, lift=0.04):
+    c = S.grid_cloth(n, 0.05, center=(0.0, 1.0, 0.0), jitter=1e-4, seed=seed, stream=7)
+    x0 = c.x.astype(np.float64)
+    p = x0.copy()
+    half = (n // 2) * 0.05
+    zc = x0[:, 2] - x0[:, 2].min()
+    fold = zc > half
+    p[fold, 2] = x0[:, 2].min() + 2 * half - zc[fold]
+    p[fold, 1] += lift
+    return c, p.astype(F32)
+
+
+def _hash_solver(cloths_and_p, table_sizes=None):
+    s = ClothSolver(use_graph=False)
+    for k, (c, p) in enumerate(cloths_and_p):
+        prm = S.Params(hash_table_size=(table_sizes[k] if table_sizes else 0),
+                       particle_diameter=float(c.diameter))
+        seg = 0 if k == 0 else s.add_segment(prm)
+        if k == 0:
+            s.set_params(prm, 0)
+        s.AddCloth(c.x, c.tris, c.diameter, seg=seg)
+        s.set_params(prm, seg)
+    for k, (c, p) in enumerate(cloths_and_p):
+        s.write_predicted(p, seg=k)
+    s.run_hash()
+    return s
+
+
+@pytest.mark.parametrize("n,M", [(16, 0), (24, 0), (16, 1), (16, 97), (40, 1 << 12)])
+def test_hash_stage_bit_exact_single(n, M):
+    say(f"hash single n={n} M={M}")
+    c, p = _folded(n, seed=n)
+    s = _hash_solver([(c, p)], [M])
+    Mx = M if M else 2 * c.n
+    cell = float(F32(F32(c.diameter) * F32(1.5)))
+    ref = oracle.hash32(p, c.x, cell, float(c.diameter), Mx)
+    _check_hash_equal(s.read_hash(0), ref, c.n)
+    s.close()
+
+
+def test_hash_stage_bit_exact_packed_segments():
+    say("hash packed segments")
+    items = [_folded(16, seed=1), _folded(33, seed=2), _folded(9, seed=3, lift=0.02), _folded(50, seed=4)]
+    sizes = [0, 3000, 0, 1 << 13]
+    s = _hash_solver(items, sizes)
+    for k, (c, p) in enumerate(items):
+        Mx = sizes[k] if sizes[k] else 2 * c.n
+        cell = float(F32(F32(c.diameter) * F32(1.5)))
+        ref = oracle.hash32(p, c.x, cell, float(c.diameter), Mx)
+        _check_hash_equal(s.read_hash(k), ref, c.n)
+    s.close()
+
+
+def test_hash_caps_and_duplicates():
+    say("hash caps/duplicates")
+    n = 80
+    x0 = np.stack([np.arange(n) * 1.0, np.zeros(n), np.zeros(n)], 1).astype(F32)
+    p = np.tile(np.array([[0.01, 0.01, 0.01]], F32), (n, 1))
+    c = S.Cloth(x=x0, tris=np.zeros((0, 3), np.uint32), stretch=np.zeros((0, 2), np.int32),
+                stretch_rest=np.zeros(0, F32), bend=np.zeros((0, 4), np.uint32), bend_rest=np.zeros(0, F32),
+                slots=np.zeros((0, 3), F32), attach_pid=np.zeros(0, np.int32), attach_slot=np.zeros(0, np.int32),
+                attach_dist=np.zeros(0, F32), diameter=F32(0.05))
+    for M in (2 * n, 1, 5):
+        s = ClothSolver(use_graph=False)
+        prm = S.Params(hash_table_size=M, particle_diameter=0.05, hash_cell_size_scalar=2.0)
+        s.set_params(prm)
+        s.AddCloth(c.x, c.tris, c.diameter)
+        s.set_params(prm)
+        s.write_predicted(p)
+        s.run_hash()
+        ref = oracle.hash32(p, x0, 0.1, 0.05, M)
+        _check_hash_equal(s.read_hash(0), ref, n)
+        s.close()
+
+
+def _run_gpu(scene, frames, frame_dt=None, use_graph=True):
+    with ClothSolver(use_graph=use_graph) as s:
+        s.load_scene(scene)
+        for _ in range(frames):
+            s.Simulate(frame_dt)
+        s.synchronize()
+        return s.positions(), s.velocities(), s.normals()
+
+
+def test_config1_one_substep_vs_fp64():
+    say("config1 one substep")
+    sc = S.config1(frames=1)
+    sc.params.num_substeps = 1
+    dt = F32(S.FRAME_DT / F32(5))
+    x, v, nrm = _run_gpu(sc, 1, frame_dt=dt)
+    ref = oracle.simulate(sc, frames=1, frame_dt=dt)
+    err = np.max(np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3)))
+    say(f"  max err {err:.3e}")
+    assert err <= 1e-5 * sc.extent()
+
+
+def test_config1_sixty_frames_vs_fp64():
+    say("config1 60 frames")
+    sc = S.config1(frames=60)
+    x, v, nrm = _run_gpu(sc, 60)
+    ref = oracle.simulate(sc, frames=60)
+    d = np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3))
+    err = float(np.max(d))
+    say(f"  max err {err:.3e} (extent {sc.extent():.3f})")
+    assert np.all(np.isfinite(x))
+    assert err <= 1e-3 * sc.extent()
+    # normals agree too (unit vectors)
+    nref = ref.normals.reshape(-1, 3)
+    assert np.max(np.abs(nrm - nref)) < 5e-2
+
+
+@pytest.mark.parametrize("bend_mode", ["quad", "pairs"])
+def test_config1_frames_reference_topologies(bend_mode):
+    say(f"config1 10 frames {bend_mode}")
+    sc = S.config1(frames=10, bend_mode=bend_mode)
+    x, _, _ = _run_gpu(sc, 10)
+    ref = oracle.simulate(sc, frames=10)
+    err = np.max(np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3)))
+    assert err <= 1e-3 * sc.extent()
+
+
+def _tiny(x, **kw):
+    from test_oracle_pins import tiny_scene
+    return tiny_scene(x, **kw)
+
+
+def test_closed_forms_through_pipeline():
+    say("closed forms")
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    # free fall with damping
+    sc = _tiny([[0, 1, 0]], num_substeps=5, damping=0.25, gravity=(0.0, -9.8, 0.0))
+    x, v, _ = _run_gpu(sc, 3)
+    ref = oracle.simulate(sc, frames=3)
+    np.testing.assert_allclose(x.reshape(-1), ref.x.reshape(-1), atol=2e-6)
+    np.testing.assert_allclose(v.reshape(-1), ref.v.reshape(-1), atol=2e-5)
+    # stretch projection: two particles 2 apart, rest 1
+    sc = _tiny([[0, 0, 0], [2, 0, 0]], stretch=[(0, 1, 1.0)])
+    x, _, _ = _run_gpu(sc, 1)
+    assert abs((x[1, 0] - x[0, 0]) - 1.0) < 2e-6
+    # attachment to 1.2 r and pinning
+    sc = _tiny([[3, 0, 0]], slots=[[0, 0, 0]], attach=[(0, 0, 1.0)])
+    x, _, _ = _run_gpu(sc, 1)
+    assert abs(np.linalg.norm(x[0]) - 1.2) < 2e-6
+    sc = _tiny([[1.0, 0, 0], [1.0, 5.0, 0]], stretch=[(0, 1, 10.0)], slots=[[0, 0, 0]],
+               attach=[(0, 0, 0.0)], num_iterations=4, relaxation_factor=1.5)
+    x, _, _ = _run_gpu(sc, 1)
+    assert abs(x[0, 0] - (1 - 1.5 / 2) ** 4) < 1e-6
+    # sphere projection
+    sc = _tiny([[0.1, 0.2, -0.05]], colliders=[S.sphere((0.0, 0.0, 0.0), 0.5)], collision_margin=0.06)
+    x, _, _ = _run_gpu(sc, 1)
+    assert abs(np.linalg.norm(x[0]) - 0.56) < 2e-6
+
+
+def test_packing_invariance_and_determinism():
+    say("batch vs alone")
+    scenes = S.config4(count=6, frames=5, smax=48)
+    alone = []
+    for sc in scenes:
+        x, v, nrm = _run_gpu(sc, 5)
+        alone.append((x, v))
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        assert s.num_segments() == len(scenes)
+        for _ in range(5):
+            s.Simulate()
+        s.synchronize()
+        for k in range(len(scenes)):
+            assert np.array_equal(s.positions(k), alone[k][0]), k
+            assert np.array_equal(s.velocities(k), alone[k][1]), k
+    # graph vs direct launches, and a second run: bit-identical
+    x1, _, _ = _run_gpu(scenes[0], 5, use_graph=False)
+    x2, _, _ = _run_gpu(scenes[0], 5, use_graph=True)
+    assert np.array_equal(x1, alone[0][0]) and np.array_equal(x2, alone[0][0])
+
+
+def test_batch_segments_vs_oracle():
+    say("batch segments vs oracle")
+    scenes = S.config4(count=4, frames=10, smax=40)
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        for _ in range(10):
+            s.Simulate()
+        s.synchronize()
+        for k, sc in enumerate(scenes):
+            ref = oracle.simulate(sc, frames=10).x.reshape(-1, 3)
+            err = np.max(np.abs(s.positions(k).astype(np.float64) - ref))
+            assert err <= 1e-3 * sc.extent(), (k, err)
+
+
+def test_normals_flat_and_host_edits():
+    say("normals / host edits")
+    sc = S.config1(frames=1, jitter=0.0)
+    sc.params.gravity = (0.0, 0.0, 0.0)
+    sc.colliders = []
+    with ClothSolver() as s:
+        s.load_scene(sc)
+        s.Simulate()
+        s.synchronize()
+        nr = s.normals()
+        assert np.all(np.abs(np.abs(nr[:, 1]) - 1) < 1e-5)
+        # MouseGrabber-style edit: pin particle 100 at a new place
+        target = np.array([0.1, 2.0, 0.1], F32)
+        s.set_particle(100, pos=target, vel=np.zeros(3, F32), inv_mass=0.0)
+        assert s.inv_masses()[100] == 0.0
+        s.Simulate()
+        s.synchronize()
+        x = s.positions()
+        assert np.linalg.norm(x[100] - target) < 0.05
+        # slot animation moves the pinned corner with it
+        s.SetAttachSlotPosition(0, sc.cloths[0].slots[0] + np.array([0, 0.05, 0], F32))
+        for _ in range(5):
+            s.Simulate()
+        s.synchronize()
+        assert s.positions()[0, 1] > sc.cloths[0].x[0, 1] + 0.02

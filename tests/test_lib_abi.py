@@ -1,0 +1,30 @@
+"""The C-ABI library loads and exports every symbol include/vtx.h declares (no GPU needed)."""
+import ctypes as C
+
+from velvet_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 30
+    missing = [name for name in declared if not hasattr(L, name)]
+    assert not missing, missing
+    # the ctypes signature table covers the header exactly
+    assert sorted(_lib._SIGS) == declared
+
+
+def test_defaults_are_the_reference_defaults():
+    p = _lib.VtxParams()
+    _lib.lib().vtx_params_default(C.byref(p))
+    assert (p.num_substeps, p.num_iterations, p.max_num_neighbors) == (2, 4, 64)
+    assert abs(p.gravity[1] + 9.8) < 1e-6 and p.damping == 0.25 and p.relaxation_factor == 1.0
+    assert abs(p.long_range_stretchiness - 1.2) < 1e-6 and abs(p.collision_margin - 0.06) < 1e-6
+    assert abs(p.friction - 0.1) < 1e-6 and p.enable_self_collision == 1 and p.interleaved_hash == 3
+    assert p.hash_cell_size_scalar == 1.5 and p.particle_diameter_scalar == 1.5
+    assert b"gfx950" in _lib.lib().vtx_version()
+
+
+def test_struct_sizes_match_header():
+    assert C.sizeof(_lib.VtxParams) == 4 * 20
+    assert C.sizeof(_lib.VtxCollider) == 4 * (1 + 3 + 3 + 16 + 16 + 1)

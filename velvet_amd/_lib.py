@@ -1,0 +1,107 @@
+"""ctypes binding of velvet_amd/libvtx.so (the C ABI of include/vtx.h).
+
+The library is the only compute path: if it is missing this module raises, there is no
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvtx.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "vtx.h")
+
+i32, u32, f32 = C.c_int32, C.c_uint32, C.c_float
+P = C.POINTER
+
+
+class VtxParams(C.Structure):
+    _fields_ = [("num_substeps", i32), ("num_iterations", i32), ("max_num_neighbors", i32),
+                ("max_speed", f32), ("gravity", f32 * 3), ("bend_compliance", f32),
+                ("damping", f32), ("relaxation_factor", f32), ("long_range_stretchiness", f32),
+                ("collision_margin", f32), ("friction", f32), ("enable_self_collision", i32),
+                ("interleaved_hash", i32), ("particle_diameter", f32),
+                ("particle_diameter_scalar", f32), ("hash_cell_size_scalar", f32),
+                ("hash_table_size", i32), ("stretch_stiffness", f32)]
+
+
+class VtxCollider(C.Structure):
+    _fields_ = [("type", i32), ("position", f32 * 3), ("scale", f32 * 3), ("cur", f32 * 16),
+                ("last", f32 * 16), ("delta_time", f32)]
+
+
+_SIGS = {
+    "vtx_params_default": (None, [P(VtxParams)]),
+    "vtx_last_error": (C.c_char_p, []),
+    "vtx_version": (C.c_char_p, []),
+    "vtx_solver_create": (i32, [P(VtxParams), P(C.c_void_p)]),
+    "vtx_solver_destroy": (i32, [C.c_void_p]),
+    "vtx_set_params": (i32, [C.c_void_p, i32, P(VtxParams)]),
+    "vtx_get_params": (i32, [C.c_void_p, i32, P(VtxParams)]),
+    "vtx_add_segment": (i32, [C.c_void_p, P(VtxParams), P(i32)]),
+    "vtx_num_segments": (i32, [C.c_void_p, P(i32)]),
+    "vtx_add_cloth": (i32, [C.c_void_p, i32, P(f32), u32, P(u32), u32, P(f32), f32, P(i32)]),
+    "vtx_add_stretch": (i32, [C.c_void_p, i32, i32, i32, f32]),
+    "vtx_add_stretch_n": (i32, [C.c_void_p, i32, P(i32), P(f32), u32]),
+    "vtx_add_attach_slot": (i32, [C.c_void_p, i32, P(f32), P(i32)]),
+    "vtx_set_attach_slot_position": (i32, [C.c_void_p, i32, i32, P(f32)]),
+    "vtx_add_attach": (i32, [C.c_void_p, i32, i32, i32, f32]),
+    "vtx_add_attach_n": (i32, [C.c_void_p, i32, P(i32), P(i32), P(f32), u32]),
+    "vtx_add_bend": (i32, [C.c_void_p, i32, u32, u32, u32, u32, f32]),
+    "vtx_add_bend_n": (i32, [C.c_void_p, i32, P(u32), P(f32), u32]),
+    "vtx_set_colliders": (i32, [C.c_void_p, i32, P(VtxCollider), u32]),
+    "vtx_simulate": (i32, [C.c_void_p, f32]),
+    "vtx_synchronize": (i32, [C.c_void_p]),
+    "vtx_set_use_graph": (i32, [C.c_void_p, i32]),
+    "vtx_set_stream": (i32, [C.c_void_p, C.c_void_p]),
+    "vtx_num_particles": (i32, [C.c_void_p, i32, P(u32)]),
+    "vtx_read_positions": (i32, [C.c_void_p, i32, P(f32)]),
+    "vtx_read_velocities": (i32, [C.c_void_p, i32, P(f32)]),
+    "vtx_read_normals": (i32, [C.c_void_p, i32, P(f32)]),
+    "vtx_read_inv_masses": (i32, [C.c_void_p, i32, P(f32)]),
+    "vtx_device_positions": (i32, [C.c_void_p, i32, P(C.c_void_p), P(u32)]),
+    "vtx_device_normals": (i32, [C.c_void_p, i32, P(C.c_void_p), P(u32)]),
+    "vtx_set_particle": (i32, [C.c_void_p, i32, i32, P(f32), P(f32), f32]),
+    "vtx_write_predicted": (i32, [C.c_void_p, i32, P(f32)]),
+    "vtx_run_hash": (i32, [C.c_void_p]),
+    "vtx_hash_table_size": (i32, [C.c_void_p, i32, P(u32)]),
+    "vtx_read_hash": (i32, [C.c_void_p, i32, P(u32), P(u32), P(u32), P(u32), P(u32)]),
+    "vtx_set_timing": (i32, [C.c_void_p, i32]),
+    "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
+}
+
+_LIB = None
+
+
+class VtxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vtx error {code}: {msg}")
+        self.code = code
+
+
+def declared_symbols():
+    """Function names declared in include/vtx.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(vtx_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python velvet_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise VtxError(rc, lib().vtx_last_error().decode())
+    return rc

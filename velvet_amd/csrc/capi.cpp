@@ -1,0 +1,272 @@
+// C ABI of the solver (declared in include/vtx.h): argument checks, error codes and the
+// per-thread error message around the C++ engine.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/vtx.h"
+#include "engine.hpp"
+
+struct vtx_solver {
+    vtx::Engine engine;
+    explicit vtx_solver(const vtx_params& p) : engine(p) {}
+};
+
+namespace {
+thread_local std::string g_error;
+void set_error(const char* m) { g_error = m ? m : ""; }
+}  // namespace
+
+#define API_BEGIN \
+    try {
+#define API_END                                   \
+    }                                             \
+    catch (const vtx::Error& e) {                 \
+        set_error(e.what());                      \
+        return e.code;                            \
+    }                                             \
+    catch (const std::bad_alloc&) {               \
+        set_error("host allocation failed");      \
+        return VTX_ENOMEM;                        \
+    }                                             \
+    catch (const std::exception& e) {             \
+        set_error(e.what());                      \
+        return VTX_EINVAL;                        \
+    }                                             \
+    return VTX_OK;
+
+#define NEED(p)                                  \
+    do {                                         \
+        if (!(p)) {                              \
+            set_error(#p " is NULL");            \
+            return VTX_EINVAL;                   \
+        }                                        \
+    } while (0)
+
+extern "C" {
+
+void vtx_params_default(vtx_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->num_substeps = 2;          // Velvet/Common.hpp:21-47
+    p->num_iterations = 4;
+    p->max_num_neighbors = 64;
+    p->max_speed = 50.f;
+    p->gravity[0] = 0.f;
+    p->gravity[1] = -9.8f;
+    p->gravity[2] = 0.f;
+    p->bend_compliance = 0.f;
+    p->damping = 0.25f;
+    p->relaxation_factor = 1.f;
+    p->long_range_stretchiness = 1.2f;
+    p->collision_margin = 0.06f;
+    p->friction = 0.1f;
+    p->enable_self_collision = 1;
+    p->interleaved_hash = 3;
+    p->particle_diameter = 0.f;
+    p->particle_diameter_scalar = 1.5f;
+    p->hash_cell_size_scalar = 1.5f;
+    p->hash_table_size = 0;
+    p->stretch_stiffness = 1.f;
+}
+
+const char* vtx_last_error(void) { return g_error.c_str(); }
+const char* vtx_version(void) { return "vtx 0.1 gfx950"; }
+
+int vtx_solver_create(const vtx_params* params, vtx_solver** out) {
+    NEED(out);
+    API_BEGIN
+    vtx_params p;
+    if (params) p = *params;
+    else vtx_params_default(&p);
+    *out = new vtx_solver(p);
+    API_END
+}
+
+int vtx_solver_destroy(vtx_solver* s) {
+    API_BEGIN
+    delete s;
+    API_END
+}
+
+int vtx_set_params(vtx_solver* s, int32_t seg, const vtx_params* params) {
+    NEED(s); NEED(params);
+    API_BEGIN s->engine.set_params(seg, *params); API_END
+}
+
+int vtx_get_params(vtx_solver* s, int32_t seg, vtx_params* out) {
+    NEED(s); NEED(out);
+    API_BEGIN *out = s->engine.seg(seg).prm; API_END
+}
+
+int vtx_add_segment(vtx_solver* s, const vtx_params* params, int32_t* seg_out) {
+    NEED(s); NEED(params);
+    API_BEGIN
+    int32_t k = s->engine.add_segment(*params);
+    if (seg_out) *seg_out = k;
+    API_END
+}
+
+int vtx_num_segments(vtx_solver* s, int32_t* out) {
+    NEED(s); NEED(out);
+    API_BEGIN *out = s->engine.num_segments(); API_END
+}
+
+int vtx_add_cloth(vtx_solver* s, int32_t seg, const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
+                  const float* model, float diameter, int32_t* offset) {
+    NEED(s);
+    API_BEGIN
+    int32_t o = s->engine.add_cloth(seg, xyz, n, tris, ntri, model, diameter);
+    if (offset) *offset = o;
+    API_END
+}
+
+int vtx_add_stretch(vtx_solver* s, int32_t seg, int32_t i, int32_t j, float rest) {
+    NEED(s);
+    int32_t ij[2] = {i, j};
+    API_BEGIN s->engine.add_stretch(seg, ij, &rest, 1); API_END
+}
+
+int vtx_add_stretch_n(vtx_solver* s, int32_t seg, const int32_t* ij, const float* rest, uint32_t count) {
+    NEED(s);
+    API_BEGIN s->engine.add_stretch(seg, ij, rest, count); API_END
+}
+
+int vtx_add_attach_slot(vtx_solver* s, int32_t seg, const float pos[3], int32_t* slot) {
+    NEED(s);
+    API_BEGIN
+    int32_t k = s->engine.add_attach_slot(seg, pos);
+    if (slot) *slot = k;
+    API_END
+}
+
+int vtx_set_attach_slot_position(vtx_solver* s, int32_t seg, int32_t slot, const float pos[3]) {
+    NEED(s);
+    API_BEGIN s->engine.set_attach_slot_position(seg, slot, pos); API_END
+}
+
+int vtx_add_attach(vtx_solver* s, int32_t seg, int32_t particle, int32_t slot, float dist) {
+    NEED(s);
+    API_BEGIN s->engine.add_attach(seg, &particle, &slot, &dist, 1); API_END
+}
+
+int vtx_add_attach_n(vtx_solver* s, int32_t seg, const int32_t* particle, const int32_t* slot, const float* dist,
+                     uint32_t count) {
+    NEED(s);
+    API_BEGIN s->engine.add_attach(seg, particle, slot, dist, count); API_END
+}
+
+int vtx_add_bend(vtx_solver* s, int32_t seg, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float angle) {
+    NEED(s);
+    uint32_t idx[4] = {i0, i1, i2, i3};
+    API_BEGIN s->engine.add_bend(seg, idx, &angle, 1); API_END
+}
+
+int vtx_add_bend_n(vtx_solver* s, int32_t seg, const uint32_t* idx, const float* angle, uint32_t count) {
+    NEED(s);
+    API_BEGIN s->engine.add_bend(seg, idx, angle, count); API_END
+}
+
+int vtx_set_colliders(vtx_solver* s, int32_t seg, const vtx_collider* cols, uint32_t n) {
+    NEED(s);
+    API_BEGIN s->engine.set_colliders(seg, cols, n); API_END
+}
+
+int vtx_simulate(vtx_solver* s, float frame_dt) {
+    NEED(s);
+    API_BEGIN s->engine.simulate(frame_dt); API_END
+}
+
+int vtx_synchronize(vtx_solver* s) {
+    NEED(s);
+    API_BEGIN s->engine.synchronize(); API_END
+}
+
+int vtx_set_use_graph(vtx_solver* s, int32_t use_graph) {
+    NEED(s);
+    API_BEGIN s->engine.set_use_graph(use_graph != 0); API_END
+}
+
+int vtx_set_stream(vtx_solver* s, void* hip_stream) {
+    NEED(s);
+    API_BEGIN s->engine.set_stream((hipStream_t)hip_stream); API_END
+}
+
+int vtx_num_particles(vtx_solver* s, int32_t seg, uint32_t* n) {
+    NEED(s); NEED(n);
+    API_BEGIN *n = s->engine.seg(seg).n(); API_END
+}
+
+int vtx_read_positions(vtx_solver* s, int32_t seg, float* xyz) {
+    NEED(s); NEED(xyz);
+    API_BEGIN s->engine.read_particles(seg, 0, xyz); API_END
+}
+
+int vtx_read_velocities(vtx_solver* s, int32_t seg, float* xyz) {
+    NEED(s); NEED(xyz);
+    API_BEGIN s->engine.read_particles(seg, 1, xyz); API_END
+}
+
+int vtx_read_normals(vtx_solver* s, int32_t seg, float* xyz) {
+    NEED(s); NEED(xyz);
+    API_BEGIN s->engine.read_particles(seg, 2, xyz); API_END
+}
+
+int vtx_read_inv_masses(vtx_solver* s, int32_t seg, float* w) {
+    NEED(s); NEED(w);
+    API_BEGIN s->engine.read_particles(seg, 3, w); API_END
+}
+
+int vtx_device_positions(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n) {
+    NEED(s); NEED(ptr);
+    API_BEGIN *ptr = (void*)s->engine.device_positions(seg, n); API_END
+}
+
+int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n) {
+    NEED(s); NEED(ptr);
+    API_BEGIN *ptr = (void*)s->engine.device_normals(seg, n); API_END
+}
+
+int vtx_set_particle(vtx_solver* s, int32_t seg, int32_t i, const float* pos, const float* vel, float inv_mass) {
+    NEED(s);
+    API_BEGIN s->engine.set_particle(seg, i, pos, vel, inv_mass); API_END
+}
+
+int vtx_write_predicted(vtx_solver* s, int32_t seg, const float* xyz) {
+    NEED(s);
+    API_BEGIN s->engine.write_predicted(seg, xyz); API_END
+}
+
+int vtx_run_hash(vtx_solver* s) {
+    NEED(s);
+    API_BEGIN s->engine.run_hash(); API_END
+}
+
+int vtx_hash_table_size(vtx_solver* s, int32_t seg, uint32_t* M) {
+    NEED(s); NEED(M);
+    API_BEGIN *M = s->engine.hash_table_size(seg); API_END
+}
+
+int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* cell_end, uint32_t* sorted_hash,
+                  uint32_t* sorted_index, uint32_t* nbr) {
+    NEED(s);
+    API_BEGIN s->engine.read_hash(seg, cell_start, cell_end, sorted_hash, sorted_index, nbr); API_END
+}
+
+int vtx_set_timing(vtx_solver* s, int32_t on) {
+    NEED(s);
+    API_BEGIN s->engine.set_timing(on != 0); API_END
+}
+
+int vtx_read_timing(vtx_solver* s, char* buf, size_t len) {
+    NEED(s); NEED(buf);
+    API_BEGIN
+    std::string r = s->engine.timing_report();
+    if (len) {
+        std::strncpy(buf, r.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    API_END
+}
+
+}  // extern "C"

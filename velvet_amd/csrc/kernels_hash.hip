@@ -1,0 +1,308 @@
+// Spatial-hash build for gfx950: cell hash, stable LSD radix sort, cell start/end and the
+// cached neighbour table (Velvet/SpatialHashGPU.cu:13-196).
+//
+// All segments are sorted in one pass sequence. Keys are the per-segment hash (< M of that
+// segment), values the local particle index. Sort tiles never straddle segments and the
+// digit histograms are laid out (segment, digit, tile), so one exclusive scan over the whole
+// histogram gives every (segment, digit, tile) its global output offset: the segments come
+// out sorted independently, in place, without sorting on segment bits. Ranking inside a tile
+// is stable (wave ballots, then waves in order, then rounds in order), so the result equals a
+// stable sort by hash within each segment -- the contract of the reference's
+// cub::DeviceRadixSort::SortPairs call (Velvet/SpatialHashGPU.cu:133-157).
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+#include "vec.hpp"
+#include "vtx_device.hpp"
+
+namespace vtx {
+
+// HashCoords (Velvet/SpatialHashGPU.cu:18-22): 32-bit wrap-around products, C remainder.
+__device__ __forceinline__ uint32_t hash_coords(int32_t x, int32_t y, int32_t z, int32_t M) {
+    const int32_t h = (int32_t)(((uint32_t)x * 92837111u) ^ ((uint32_t)y * 689287499u) ^
+                                ((uint32_t)z * 283923481u));
+    const int32_t r = h % M;
+    return (uint32_t)(r < 0 ? -r : r);
+}
+
+// ComputeIntCoord (Velvet/SpatialHashGPU.cu:13-16): floor of a correctly rounded binary32
+// quotient (the library is built with -fhip-fp32-correctly-rounded-divide-sqrt).
+__device__ __forceinline__ int32_t int_coord(float v, float cell) {
+    return (int32_t)floorf(__fdiv_rn(v, cell));
+}
+
+// ComputeParticleHash_Kernel (Velvet/SpatialHashGPU.cu:34-42) over predicted positions.
+__global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
+    const TileDev t = d.tiles[blockIdx.x];
+    if (threadIdx.x >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    const uint32_t g = t.begin + threadIdx.x;
+    d.vals[0][g] = g - sg.pbase;
+    if (!(sg.flags & VTX_SEG_SELFCOL)) {
+        d.keys[0][g] = 0;      // segment without self-collision: sorted as one cell, unused
+        return;
+    }
+    const float4 p = d.P[in][g];
+    d.keys[0][g] = hash_coords(int_coord(p.x, sg.cell), int_coord(p.y, sg.cell),
+                               int_coord(p.z, sg.cell), (int32_t)sg.M);
+}
+
+// Per-tile digit histogram into hist[hbase + digit * ntiles + tidx].
+__global__ __launch_bounds__(VTX_TILE) void k_sort_hist(DevState d, int src, int shift) {
+    __shared__ uint32_t cnt[256];
+    const SortTileDev t = d.stiles[blockIdx.x];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t* __restrict__ keys = d.keys[src] + t.begin;
+    for (uint32_t r = 0; r < VTX_SORT_ITEMS; r++) {
+        const uint32_t i = r * VTX_TILE + threadIdx.x;
+        if (i < t.count) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    d.hist[t.hbase + threadIdx.x * t.ntiles + t.tidx] = cnt[threadIdx.x];
+}
+
+// Wave-level exclusive scan helpers (wave64).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// Block (256 threads) exclusive scan; returns the exclusive prefix, *total the block sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total, uint32_t* sh) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < VTX_TILE / 64; w++) {
+        const uint32_t s = sh[w];
+        off += (w < wave) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// Device-wide exclusive scan of hist[0..n): block sums, scan of sums, apply.
+__global__ __launch_bounds__(VTX_TILE) void k_scan_reduce(uint32_t* __restrict__ a, uint32_t n,
+                                                          uint32_t* __restrict__ part) {
+    __shared__ uint32_t sh[VTX_TILE / 64];
+    const uint32_t base = blockIdx.x * VTX_SCAN_BLOCK;
+    uint32_t s = 0;
+    for (uint32_t r = 0; r < VTX_SCAN_BLOCK / VTX_TILE; r++) {
+        const uint32_t i = base + r * VTX_TILE + threadIdx.x;
+        if (i < n) s += a[i];
+    }
+    uint32_t total;
+    block_excl_scan(s, &total, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(VTX_TILE) void k_scan_partials(uint32_t* __restrict__ part, uint32_t nb) {
+    __shared__ uint32_t sh[VTX_TILE / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += VTX_TILE) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? part[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, &total, sh);
+        if (i < nb) part[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(VTX_TILE) void k_scan_apply(uint32_t* __restrict__ a, uint32_t n,
+                                                         const uint32_t* __restrict__ part) {
+    __shared__ uint32_t sh[VTX_TILE / 64];
+    const uint32_t base = blockIdx.x * VTX_SCAN_BLOCK;
+    uint32_t carry = part[blockIdx.x];
+    for (uint32_t r = 0; r < VTX_SCAN_BLOCK / VTX_TILE; r++) {
+        const uint32_t i = base + r * VTX_TILE + threadIdx.x;
+        const uint32_t v = i < n ? a[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, &total, sh);
+        if (i < n) a[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// Stable scatter of one sort tile: rounds of 256 keys in order; inside a round, ranks come
+// from a wave match on the 8 digit bits (ballots) and the waves' per-digit counts.
+__global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, int shift) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[VTX_TILE / 64][256];
+    const SortTileDev t = d.stiles[blockIdx.x];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    base[tid] = d.hist[t.hbase + tid * t.ntiles + t.tidx];
+    const uint32_t* __restrict__ kin = d.keys[src] + t.begin;
+    const uint32_t* __restrict__ vin = d.vals[src] + t.begin;
+    uint32_t* __restrict__ kout = d.keys[src ^ 1];
+    uint32_t* __restrict__ vout = d.vals[src ^ 1];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t r = 0; r < VTX_SORT_ITEMS; r++) {
+        const uint32_t i = r * VTX_TILE + tid;
+        const bool valid = i < t.count;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+#pragma unroll
+        for (uint32_t w = 0; w < VTX_TILE / 64; w++) wcnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (dg >> b) & 1u;
+            const uint64_t bal = __ballot(valid && bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = base[tid];
+#pragma unroll
+            for (uint32_t w = 0; w < VTX_TILE / 64; w++) {
+                const uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += c;
+            }
+            base[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = wcnt[wave][dg] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+// FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) per segment, with local sorted
+// positions; empty cells keep the 0xffffffff written by the fill.
+__global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int src) {
+    const TileDev t = d.tiles[blockIdx.x];
+    if (threadIdx.x >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    if (!(sg.flags & VTX_SEG_SELFCOL)) return;
+    const uint32_t g = t.begin + threadIdx.x;
+    const uint32_t local = g - sg.pbase;
+    const uint32_t* __restrict__ sk = d.keys[src];
+    const uint32_t key = sk[g];
+    if (local == 0) {
+        d.cell_start[sg.tbase + key] = 0;
+    } else {
+        const uint32_t prev = sk[g - 1];
+        if (key != prev) {
+            d.cell_start[sg.tbase + key] = local;
+            d.cell_end[sg.tbase + prev] = local;
+        }
+    }
+    if (local == sg.n - 1) d.cell_end[sg.tbase + key] = sg.n;
+}
+
+// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130), one thread per particle in index
+// order (so the column-major table writes coalesce). Cells are visited x-, y-, z-nested;
+// inside a cell in sorted order, at most max_nbr entries per cell and max_nbr in total; the
+// list ends with 0xffffffff unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
+__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
+#pragma clang fp contract(off)
+    const TileDev t = d.tiles[blockIdx.x];
+    if (threadIdx.x >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    if (!(sg.flags & VTX_SEG_SELFCOL)) return;
+    const uint32_t g = t.begin + threadIdx.x;
+    const uint32_t id = g - sg.pbase;
+    const uint32_t n = sg.n;
+    const float4* __restrict__ P = d.P[in];
+    const float4* __restrict__ X0 = d.X0;
+    const uint32_t* __restrict__ sv = d.vals[src] + sg.pbase;
+    const uint32_t* __restrict__ cs = d.cell_start + sg.tbase;
+    const uint32_t* __restrict__ ce = d.cell_end + sg.tbase;
+    uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
+    const float4 p = P[g];
+    const float4 o = X0[g];
+    const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
+    const int32_t M = (int32_t)sg.M;
+    const uint32_t K = sg.max_nbr;
+    uint32_t k = 0;
+    for (int32_t xx = ix - 1; xx <= ix + 1; xx++)
+        for (int32_t yy = iy - 1; yy <= iy + 1; yy++)
+            for (int32_t zz = iz - 1; zz <= iz + 1; zz++) {
+                const uint32_t h = hash_coords(xx, yy, zz, M);
+                const uint32_t start = cs[h];
+                if (start == VTX_INVALID) continue;
+                const uint32_t end = min(ce[h], start + K);
+                for (uint32_t s = start; s < end; s++) {
+                    const uint32_t j = sv[s];
+                    if (j == id) continue;
+                    const float4 q = P[sg.pbase + j];
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (!(d2 < sg.cell2)) continue;
+                    const float4 r = X0[sg.pbase + j];
+                    const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
+                    const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+                    if (!(e2 > sg.diam2)) continue;
+                    nb[(size_t)k * n] = j;
+                    k++;
+                    if (k >= K) return;
+                }
+            }
+    nb[(size_t)k * n] = VTX_INVALID;
+}
+
+// ---------------------------------------------------------------- host launchers
+hipError_t launch_hash_keys(const DevState& d, int in, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash_keys, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s) {
+    const uint32_t nb = (n + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK;
+    if (!nb) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(VTX_TILE), 0, s, a, n, part);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(VTX_TILE), 0, s, part, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(VTX_TILE), 0, s, a, n, (const uint32_t*)part);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_hist, dim3(d.nstiles), dim3(VTX_TILE), 0, s, d, src, shift);
+    hipError_t e = launch_scan(d.hist, d.nhist, d.part, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sort_scatter, dim3(d.nstiles), dim3(VTX_TILE), 0, s, d, src, shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s) {
+    if (!n) return hipSuccess;
+    uint32_t blocks = (n + 1023) / 1024;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(1024), 0, s, a, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_bounds(const DevState& d, int src, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_bounds, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, src);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
+    return hipGetLastError();
+}
+
+}  // namespace vtx

@@ -1,0 +1,24 @@
+// Host-side launchers of the HIP kernels (one per pass); all are asynchronous on the stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "vtx_device.hpp"
+
+namespace vtx {
+
+enum { EPILOGUE_NONE = 0, EPILOGUE_FINALIZE = 1, EPILOGUE_FINALIZE_PREDICT = 2 };
+
+hipError_t launch_frame_begin(const DevState& d, float frame_dt, float h, hipStream_t s);
+hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s);
+hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints,
+                          hipStream_t s);
+hipError_t launch_normals(const DevState& d, hipStream_t s);
+
+hipError_t launch_hash_keys(const DevState& d, int in, hipStream_t s);
+hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s);
+hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
+hipError_t launch_cell_bounds(const DevState& d, int src, hipStream_t s);
+hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s);
+
+}  // namespace vtx

@@ -1,0 +1,93 @@
+// Device-side data layout shared by the HIP kernels and the host engine.
+//
+// Particles of all segments are packed back to back in float4 arrays (xyz + one payload lane):
+//   X  positions          w = inverse mass
+//   V  velocities         w unused
+//   P  predicted (x2)     w = inverse mass, ping-ponged between Jacobi passes
+//   X0 initial positions  collision filter snapshot (Velvet/SpatialHashGPU.hpp:32-39)
+//   N  normals
+// Work is cut into particle tiles of VTX_TILE consecutive particles of one segment, one tile
+// per workgroup, so every per-segment value a kernel needs is wave-uniform (scalar loads).
+// Incident constraints are stored per tile in sliced-ELL form: entry k of lane l sits at
+// [tile offset + k * VTX_TILE + l], so the lanes of a wave read consecutive words.
+#pragma once
+#include <stdint.h>
+
+#define VTX_TILE 256
+#define VTX_SORT_ITEMS 8                       // keys per thread in a sort tile
+#define VTX_SORT_TILE (VTX_TILE * VTX_SORT_ITEMS)
+#define VTX_SCAN_BLOCK 4096                     // elements per block of the device scan
+#define VTX_INVALID 0xFFFFFFFFu
+#define VTX_EPSILON 1e-6f                       // Velvet/Common.cuh:21
+#define VTX_MAX_NEIGHBORS 64
+
+#define VTX_SEG_SELFCOL 1u
+
+namespace vtx {
+
+struct alignas(16) SegDev {
+    uint32_t pbase, n, tbase, M;           // particle base / count, hash-table base / size
+    uint32_t nbase_lo, nbase_hi, sbase, scount;   // neighbour-table base, attach-slot base/count
+    uint32_t cbase, ccount, flags, max_nbr;       // collider base/count, flags, neighbour cap
+    float diameter, cell, cell2, diam2;
+    float max_speed, damping, omega, lrs;
+    float bend_compliance, margin, friction, stiffness;
+    float gx, gy, gz, pad0;
+};
+
+struct alignas(16) TileDev {
+    uint32_t seg, begin, count, pad0;      // begin = global index of the tile's first particle
+    uint32_t st_off, st_k, at_off, at_k;   // stretch / attach ELL offsets (entries) and widths
+    uint32_t bd_off, bd_k, tr_off, tr_k;   // bend / triangle ELL offsets and widths
+};
+
+struct alignas(16) SortTileDev {
+    uint32_t seg, begin, count, hbase;     // hbase: histogram base of the segment
+    uint32_t ntiles, tidx, pad0, pad1;     // sort tiles of the segment, index of this one
+};
+
+// Collider with its transforms as 3x4 row-major affine matrices:
+//   inv = inverse(cur), rel = last * inverse(cur) (Velvet/VtClothSolverGPU.cuh:91-96),
+//   cur3 = upper-left 3x3 of cur (used by the cube's correction, cuh:86).
+struct alignas(16) ColliderDev {
+    int32_t type;
+    float dt;
+    float pad0, pad1;
+    float pos[4];
+    float scale[4];
+    float cur3[12];
+    float inv[12];
+    float rel[12];
+};
+
+// Pointers and sizes every kernel may need; passed by value (kernel-argument segment).
+struct DevState {
+    const SegDev* segs;
+    const TileDev* tiles;
+    const SortTileDev* stiles;
+    float4* X;
+    float4* V;
+    float4* P[2];
+    const float4* X0;
+    float4* Nrm;
+    const uint2* st;          // stretch ELL: (other particle, rest bits)
+    const uint2* at;          // attach ELL: (slot, distance bits)
+    const uint32_t* bd;       // bend ELL: bend id | role << 30
+    const uint4* brec;        // bend records: 4 global particle indices
+    const float* brest;       // bend rest angles
+    const uint32_t* tr;       // triangle ELL: triangle id
+    const uint4* trec;        // triangle records: 3 global particle indices
+    const float4* slots;      // attach slot positions
+    const ColliderDev* cols;
+    uint32_t* keys[2];        // sort keys (hash) ping-pong
+    uint32_t* vals[2];        // sort values (local particle index) ping-pong
+    uint32_t* hist;           // per (segment, digit, sort tile) counts, scanned in place
+    uint32_t* part;           // scan partials
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    uint32_t* nbr;            // neighbour tables, column-major per segment
+    uint32_t ntiles, nstiles, nhist, ncells;
+    uint32_t nparticles, pad0, pad1, pad2;
+};
+
+}  // namespace vtx
