@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-substeps/s of the XPBD cloth solver on one MI355X.
+
+Default workload (``--workload batch512``): 512 independent cloths, sides log-uniform 16..128
+(seeded), each with its own stiffness, spacing, 0-4 attached corners and 1-3 sphere colliders,
+5 substeps x 10 iterations, self-collision on. Batched = every cloth a segment of one solver,
+one launch per stage, each frame replayed as a hipGraph. Plain = one solver per cloth, each
+launching its own kernels per stage on one shared stream, no graph.
+
+A step is one 1/60 s frame of every cloth. ``value`` = sum(particles) x substeps x steps /
+seconds of the timed region (inputs resident in HBM). The line also carries the plain path's
+rate, per-frame latency percentiles, the dominant kernel's roofline (HIP events on the
+solver's stream) and the CPU baseline (the restated reference CPU solver on host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from velvet_amd import scenes as S  # noqa: E402
+from velvet_amd.solver import ClothSolver  # noqa: E402
+
+METRIC = "particle-substeps/s, batched vs plain; p50/p99 step latency at fixed QPS"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def hip_stream():
+    hip = ctypes.CDLL("libamdhip64.so")
+    s = ctypes.c_void_p()
+    rc = hip.hipStreamCreateWithFlags(ctypes.byref(s), ctypes.c_uint(1))
+    if rc != 0:
+        raise RuntimeError(f"hipStreamCreateWithFlags failed: {rc}")
+    return s.value
+
+
+def workload_scenes(name):
+    if name == "batch512":
+        return S.config4(count=512), "512 cloths, sides log-uniform 16..128, 5x10, per-cloth hash grids"
+    if name == "config1":
+        return [S.config1()], "40x40 cloth, 2 pinned corners, sphere, 5x10, omega 1.5"
+    if name == "config2":
+        return [S.config2()], "256x256 cloth, 256-slot LRA edge, sphere + plane, 2^18 cells, cell = d, 5x10"
+    if name == "config3":
+        return [S.config3()], "1024x1024 cloth, 3 seeded spheres, 2^21 cells, 4x8"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def counts(scenes):
+    tot = dict(N=0, S=0, B=0, A=0, L=0, T=0)
+    for sc in scenes:
+        c = sc.counts()
+        for k in tot:
+            tot[k] += c[k]
+    return tot
+
+
+def time_frames(solver, frames):
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        solver.Simulate()
+    solver.synchronize()
+    return time.perf_counter() - t0
+
+
+def latency(solver, frames):
+    out = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        solver.Simulate()
+        solver.synchronize()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out
+
+
+def kernel_times(solver, frames):
+    solver.set_timing(True)
+    for _ in range(frames):
+        solver.Simulate()
+    solver.synchronize()
+    rep = json.loads(solver.timing())
+    solver.set_timing(False)
+    return rep
+
+
+def plain_rate(scenes, frames, stream):
+    solvers = []
+    for sc in scenes:
+        s = ClothSolver(use_graph=False)
+        s.load_scene(sc)
+        s.set_stream(stream)
+        solvers.append(s)
+    for s in solvers:       # warm-up frame (commit + first launches)
+        s.Simulate()
+    solvers[0].synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        for s in solvers:
+            s.Simulate()
+    solvers[0].synchronize()
+    dt = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    return dt
+
+
+def cpu_baseline(scenes, budget_s):
+    import oracle
+    done_ps, spent, frames, cloths = 0, 0.0, 0, 0
+    k = 0
+    while spent < budget_s and k < 4 * len(scenes):
+        sc = scenes[k % len(scenes)]
+        cpu = oracle.CpuSolver(sc)
+        cpu.simulate()                     # warm-up frame, excluded
+        t0 = time.perf_counter()
+        cpu.simulate()
+        dt = time.perf_counter() - t0
+        spent += dt
+        done_ps += sc.n * sc.params.num_substeps
+        frames += 1
+        cloths += 1
+        k += 1
+        del cpu
+    return done_ps / spent, spent, frames, cloths
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="batch512")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--latency-steps", type=int, default=30)
+    ap.add_argument("--kernel-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the batched "
+                                   "and served jobs shard across GPUs with no exchange)"}))
+        sys.exit(2)
+
+    scenes, desc = workload_scenes(args.workload)
+    tot = counts(scenes)
+    substeps = scenes[0].params.num_substeps
+    iters = scenes[0].params.num_iterations
+    log(f"workload {args.workload}: {len(scenes)} cloths, {tot}")
+
+    solver = ClothSolver(use_graph=True)
+    for sc in scenes:
+        solver.load_scene(sc)
+    log("registered; warming up")
+    for _ in range(args.warmup):
+        solver.Simulate()
+    solver.synchronize()
+    log(f"timing {args.steps} frames")
+    dt = time_frames(solver, args.steps)
+    value = tot["N"] * substeps * args.steps / dt
+    ms_per_step = dt / args.steps * 1e3
+    log(f"batched: {value:.4e} particle-substeps/s, {ms_per_step:.3f} ms/frame")
+
+    lat = latency(solver, args.latency_steps)
+    kt = kernel_times(solver, args.kernel_steps)
+    it = kt.get("k_iterate", {"ms": float("nan"), "launches": 1})
+    avg_ms = it["ms"] / max(1, it["launches"])
+    alg_bytes = S.algorithmic_bytes_per_iteration(tot)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_iterate", "avg_launch_ms": round(avg_ms, 6),
+                "algorithmic_bytes_per_launch": alg_bytes}
+    solver.close()
+
+    plain = None
+    if not args.no_plain and len(scenes) > 1:
+        log(f"plain path: {len(scenes)} solvers, {args.plain_steps} frames")
+        pdt = plain_rate(scenes, args.plain_steps, hip_stream())
+        pval = tot["N"] * substeps * args.plain_steps / pdt
+        plain = {"value": pval, "ms_per_step": pdt / args.plain_steps * 1e3,
+                 "batched_over_plain": value / pval}
+        log(f"plain: {pval:.4e} particle-substeps/s -> batched/plain {value / pval:.2f}x")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline ({args.cpu_seconds:.0f} s budget)")
+        rate, spent, frames, cloths = cpu_baseline(scenes, args.cpu_seconds)
+        cpu = {"value": rate, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+               "sample": f"restated reference CPU solver (Gauss-Seidel, VtClothSolverCPU + "
+                         f"SpatialHashCPU), single thread, 1 timed frame each of the first {cloths} "
+                         f"cloths of the workload after a warm-up frame, {spent:.1f} s",
+               "host_cpu": _cpu_model()}
+
+    line = {"metric": METRIC, "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded cloths)",
+            "config": {"workload": f"{args.workload}: {desc}", "particles": tot["N"],
+                       "stretch": tot["S"], "bend": tot["B"], "attach": tot["A"],
+                       "substeps": substeps, "iterations": iters, "graph": True},
+            "plain": plain,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "frames": len(lat)},
+            "kernels_ms": {k: v for k, v in kt.items()},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip() + f" ({os.cpu_count()} logical)"
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

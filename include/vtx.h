@@ -167,9 +167,11 @@ int vtx_hash_table_size(vtx_solver* s, int32_t seg, uint32_t* M);
 int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* cell_end,
                   uint32_t* sorted_hash, uint32_t* sorted_index, uint32_t* nbr);
 
-/* Per-stage GPU time of the last frames, in ms, accumulated since the last reset, by the
- * reference's timer labels (Velvet/GUI.cpp:32-51). Only filled when timing is on; timing
- * turns the graph off. */
+/* Per-kernel GPU time, the counterpart of the reference's ScopedTimerGPU labels
+ * (Velvet/Timer.hpp:242-260, Velvet/GUI.cpp:32-51). With timing on, frames are launched
+ * without the graph and every launch group is bracketed by a HIP event pair on the solver's
+ * stream; vtx_read_timing returns JSON {label: {"ms": total, "launches": count}} accumulated
+ * since timing was switched on. */
 int vtx_set_timing(vtx_solver* s, int32_t on);
 int vtx_read_timing(vtx_solver* s, char* buf, size_t len);
 

@@ -578,56 +578,59 @@ void Engine::stage_end(hipStream_t s) {
 
 // SpatialHashGPU::Hash -> HashObjects (Velvet/SpatialHashGPU.hpp:41-52, Velvet/SpatialHashGPU.cu:159-196).
 void Engine::hash_build(int in, hipStream_t s) {
-    stage_begin("Solver_HashParticle", s);
+    stage_begin("k_hash_keys", s);
     VTX_HIP(launch_hash_keys(ds_, in, s));
     stage_end(s);
-    stage_begin("Solver_HashSort", s);
+    stage_begin("hash_sort", s);
     int src = 0;
     for (int pass = 0; pass < passes_; pass++) {
         VTX_HIP(launch_sort_pass(ds_, src, 8 * pass, s));
         src ^= 1;
     }
     stage_end(s);
-    stage_begin("Solver_HashBuildCell", s);
+    stage_begin("hash_cells", s);
     VTX_HIP(launch_fill_u32(ds_.cell_start, ds_.ncells, VTX_INVALID, s));
     VTX_HIP(launch_cell_bounds(ds_, src, s));
     stage_end(s);
-    stage_begin("Solver_HashCache", s);
+    stage_begin("k_neighbors", s);
     VTX_HIP(launch_neighbors(ds_, in, src, s));
     stage_end(s);
     sorted_src_ = src;
 }
 
-// VtClothSolverGPU::Simulate stage order (Velvet/VtClothSolverGPU.hpp:56-111).
+// VtClothSolverGPU::Simulate stage order (Velvet/VtClothSolverGPU.hpp:56-111). In timing mode
+// every launch group is bracketed by an event pair labelled with its kernel.
 void Engine::record_frame(float frame_dt, hipStream_t s) {
     const float h = frame_dt / (float)substeps_;
-    stage_begin("Solver_Predict", s);
+    stage_begin("k_frame_begin", s);
     VTX_HIP(launch_frame_begin(ds_, frame_dt, h, s));
     stage_end(s);
     int cur = 0;
     for (int sub = 0; sub < substeps_; sub++) {
         if (any_selfcol_ && sub % interleave_ == 0) hash_build(cur, s);
         if (need_collide_) {
-            stage_begin("Solver_CollideParticles", s);
+            stage_begin("k_collide", s);
             VTX_HIP(launch_collide(ds_, cur, h, s));
             stage_end(s);
             cur ^= 1;
         }
         const bool last_sub = sub == substeps_ - 1;
         const int epi_last = last_sub ? EPILOGUE_FINALIZE : EPILOGUE_FINALIZE_PREDICT;
-        stage_begin("Solver_SolveConstraints", s);
         if (iterations_ == 0) {
+            stage_begin("k_iterate", s);
             VTX_HIP(launch_iterate(ds_, cur, h, epi_last, false, s));
+            stage_end(s);
             cur ^= 1;
         }
         for (int it = 0; it < iterations_; it++) {
             const int epi = it == iterations_ - 1 ? epi_last : EPILOGUE_NONE;
+            stage_begin("k_iterate", s);
             VTX_HIP(launch_iterate(ds_, cur, h, epi, true, s));
+            stage_end(s);
             cur ^= 1;
         }
-        stage_end(s);
     }
-    stage_begin("Solver_UpdateNormals", s);
+    stage_begin("k_normals", s);
     VTX_HIP(launch_normals(ds_, s));
     stage_end(s);
 }
@@ -644,7 +647,9 @@ void Engine::simulate(float frame_dt) {
             float ms = 0.f;
             VTX_HIP(hipEventElapsedTime(&ms, stages_[k].a, stages_[k].b));
             timing_ms_[stages_[k].label] += ms;
-            timing_ms_["Solver_Total"] += ms;
+            timing_n_[stages_[k].label] += 1;
+            timing_ms_["total"] += ms;
+            timing_n_["total"] += 1;
         }
     } else if (use_graph_) {
         if (!gexec_ || graph_dt_ != frame_dt) {
@@ -703,14 +708,17 @@ void Engine::set_stream(hipStream_t s) {
 void Engine::set_timing(bool on) {
     timing_ = on;
     timing_ms_.clear();
+    timing_n_.clear();
 }
 
 std::string Engine::timing_report() {
     std::ostringstream o;
     o << "{";
     bool first = true;
+    o.precision(9);
     for (auto& kv : timing_ms_) {
-        o << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
+        o << (first ? "" : ", ") << "\"" << kv.first << "\": {\"ms\": " << kv.second
+          << ", \"launches\": " << timing_n_[kv.first] << "}";
         first = false;
     }
     o << "}";

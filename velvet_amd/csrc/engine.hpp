@@ -137,6 +137,7 @@ private:
     std::vector<StageTime> stages_;
     size_t stage_used_ = 0;
     std::map<std::string, double> timing_ms_;
+    std::map<std::string, long> timing_n_;
 };
 
 }  // namespace vtx
