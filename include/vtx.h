@@ -133,6 +133,11 @@ int vtx_synchronize(vtx_solver* s);
 /* Launch-mode switches. use_graph: record each frame's launches into a hipGraph and replay
  * it (default 1). */
 int vtx_set_use_graph(vtx_solver* s, int32_t use_graph);
+/* Extension: run the Jacobi pass of segments whose cloth is a regular vertex grid with the
+ * reference topology on 2-D LDS tiles (default 1); other segments use per-particle incidence
+ * lists. vtx_stencil_segments reports how many segments took the stencil layout. */
+int vtx_set_stencil_path(vtx_solver* s, int32_t enable);
+int vtx_stencil_segments(vtx_solver* s, int32_t* count);
 /* Bind the solver to an existing stream (hipStream_t), or NULL for the solver's own. */
 int vtx_set_stream(vtx_solver* s, void* hip_stream);
 

@@ -113,31 +113,35 @@ def test_hash_caps_and_duplicates():
         s.close()
 
 
-def _run_gpu(scene, frames, frame_dt=None, use_graph=True):
-    with ClothSolver(use_graph=use_graph) as s:
+def _run_gpu(scene, frames, frame_dt=None, use_graph=True, stencil=True, expect_stencil=None):
+    with ClothSolver(use_graph=use_graph, stencil=stencil) as s:
         s.load_scene(scene)
+        if expect_stencil is not None:
+            assert s.stencil_segments() == expect_stencil
         for _ in range(frames):
             s.Simulate(frame_dt)
         s.synchronize()
         return s.positions(), s.velocities(), s.normals()
 
 
-def test_config1_one_substep_vs_fp64():
-    say("config1 one substep")
+@pytest.mark.parametrize("stencil", [True, False])
+def test_config1_one_substep_vs_fp64(stencil):
+    say(f"config1 one substep stencil={stencil}")
     sc = S.config1(frames=1)
     sc.params.num_substeps = 1
     dt = F32(S.FRAME_DT / F32(5))
-    x, v, nrm = _run_gpu(sc, 1, frame_dt=dt)
+    x, v, nrm = _run_gpu(sc, 1, frame_dt=dt, stencil=stencil, expect_stencil=int(stencil))
     ref = oracle.simulate(sc, frames=1, frame_dt=dt)
     err = np.max(np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3)))
     say(f"  max err {err:.3e}")
     assert err <= 1e-5 * sc.extent()
 
 
-def test_config1_sixty_frames_vs_fp64():
-    say("config1 60 frames")
+@pytest.mark.parametrize("stencil", [True, False])
+def test_config1_sixty_frames_vs_fp64(stencil):
+    say(f"config1 60 frames stencil={stencil}")
     sc = S.config1(frames=60)
-    x, v, nrm = _run_gpu(sc, 60)
+    x, v, nrm = _run_gpu(sc, 60, stencil=stencil, expect_stencil=int(stencil))
     ref = oracle.simulate(sc, frames=60)
     d = np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3))
     err = float(np.max(d))
@@ -149,11 +153,12 @@ def test_config1_sixty_frames_vs_fp64():
     assert np.max(np.abs(nrm - nref)) < 5e-2
 
 
-@pytest.mark.parametrize("bend_mode", ["quad", "pairs"])
-def test_config1_frames_reference_topologies(bend_mode):
-    say(f"config1 10 frames {bend_mode}")
+@pytest.mark.parametrize("bend_mode", ["quad", "pairs", "none"])
+@pytest.mark.parametrize("stencil", [True, False])
+def test_config1_frames_reference_topologies(bend_mode, stencil):
+    say(f"config1 10 frames {bend_mode} stencil={stencil}")
     sc = S.config1(frames=10, bend_mode=bend_mode)
-    x, _, _ = _run_gpu(sc, 10)
+    x, _, _ = _run_gpu(sc, 10, stencil=stencil, expect_stencil=int(stencil))
     ref = oracle.simulate(sc, frames=10)
     err = np.max(np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3)))
     assert err <= 1e-3 * sc.extent()
@@ -254,3 +259,43 @@ def test_normals_flat_and_host_edits():
             s.Simulate()
         s.synchronize()
         assert s.positions()[0, 1] > sc.cloths[0].x[0, 1] + 0.02
+
+
+def test_stencil_detection_rejects_non_grid():
+    say("stencil detection")
+    sc = S.config1(frames=1)
+    c = sc.cloths[0]
+    # an extra long-range stretch pair is not a grid neighbour: general path
+    c2 = S.Cloth(**{**c.__dict__, "stretch": np.concatenate([c.stretch, [[0, 45]]]).astype(np.int32),
+                   "stretch_rest": np.concatenate([c.stretch_rest, [np.float32(0.3)]]).astype(F32)})
+    sc2 = S.Scene(sc.params, [c2], sc.colliders, 1)
+    with ClothSolver() as s:
+        s.load_scene(sc2)
+        assert s.stencil_segments() == 0
+        s.Simulate()
+        s.synchronize()
+        x = s.positions()
+    ref = oracle.simulate(sc2, frames=1).x.reshape(-1, 3)
+    assert np.max(np.abs(x - ref)) <= 1e-3 * sc.extent()
+    # two cloths in one segment (the reference's shared solver): general path, shared hash
+    sc3 = S.Scene(sc.params, [c, S.grid_cloth(12, 0.05, center=(0.0, 1.8, 0.0), seed=5, stream=9)],
+                  sc.colliders, 1)
+    with ClothSolver() as s:
+        s.load_scene(sc3)
+        assert s.stencil_segments() == 0 and s.num_particles() == c.n + 144
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        x = s.positions()
+    ref = oracle.simulate(sc3, frames=3).x.reshape(-1, 3)
+    assert np.max(np.abs(x - ref)) <= 1e-3 * sc.extent()
+
+
+def test_config2_lra_frames_vs_fp64():
+    say("config2 3 frames")
+    sc = S.config2(frames=3)
+    x, _, _ = _run_gpu(sc, 3, expect_stencil=1)
+    ref = oracle.simulate(sc, frames=3).x.reshape(-1, 3)
+    err = np.max(np.abs(x.astype(np.float64) - ref))
+    say(f"  max err {err:.3e}")
+    assert err <= 1e-3 * sc.extent()

@@ -56,6 +56,8 @@ _SIGS = {
     "vtx_synchronize": (i32, [C.c_void_p]),
     "vtx_set_use_graph": (i32, [C.c_void_p, i32]),
     "vtx_set_stream": (i32, [C.c_void_p, C.c_void_p]),
+    "vtx_set_stencil_path": (i32, [C.c_void_p, i32]),
+    "vtx_stencil_segments": (i32, [C.c_void_p, P(i32)]),
     "vtx_num_particles": (i32, [C.c_void_p, i32, P(u32)]),
     "vtx_read_positions": (i32, [C.c_void_p, i32, P(f32)]),
     "vtx_read_velocities": (i32, [C.c_void_p, i32, P(f32)]),

@@ -187,6 +187,16 @@ int vtx_set_use_graph(vtx_solver* s, int32_t use_graph) {
     API_BEGIN s->engine.set_use_graph(use_graph != 0); API_END
 }
 
+int vtx_set_stencil_path(vtx_solver* s, int32_t enable) {
+    NEED(s);
+    API_BEGIN s->engine.set_stencil_path(enable != 0); API_END
+}
+
+int vtx_stencil_segments(vtx_solver* s, int32_t* count) {
+    NEED(s); NEED(count);
+    API_BEGIN *count = s->engine.stencil_segments(); API_END
+}
+
 int vtx_set_stream(vtx_solver* s, void* hip_stream) {
     NEED(s);
     API_BEGIN s->engine.set_stream((hipStream_t)hip_stream); API_END

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <sstream>
 
 #include "launch.hpp"
@@ -129,6 +130,7 @@ int32_t Engine::add_cloth(int32_t s, const float* xyz, uint32_t n, const uint32_
         sg.v.insert(sg.v.end(), {0.f, 0.f, 0.f});
         sg.w.push_back(1.0f);
     }
+    sg.cloth_starts.push_back(prev);
     sg.x0 = sg.x;   // SetInitialPositions snapshot of every particle (Velvet/SpatialHashGPU.hpp:32-39)
     mark_dirty();
     return (int32_t)prev;
@@ -328,10 +330,135 @@ static void pack_tile(std::vector<E>& out, const std::vector<uint32_t>& start, c
     }
 }
 
+// Sliced-ELL packing with an explicit lane -> particle map (-1: lane has no particle).
+template <class E>
+static void pack_lanes(std::vector<E>& out, const std::vector<uint32_t>& start, const std::vector<E>& items,
+                       const std::vector<int64_t>& lanes, E pad, uint32_t* off, uint32_t* width) {
+    uint32_t K = 0;
+    for (int64_t i : lanes)
+        if (i >= 0) K = std::max(K, start[(size_t)i + 1] - start[(size_t)i]);
+    *off = (uint32_t)out.size();
+    *width = K;
+    out.resize(out.size() + (size_t)K * VTX_TILE, pad);
+    for (size_t l = 0; l < lanes.size(); l++) {
+        if (lanes[l] < 0) continue;
+        const uint32_t b = start[(size_t)lanes[l]], e = start[(size_t)lanes[l] + 1];
+        for (uint32_t k = 0; k < e - b; k++) out[*off + (size_t)k * VTX_TILE + l] = items[b + k];
+    }
+}
+
 static inline uint32_t fbits(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
     return u;
+}
+
+// ---------------------------------------------------------------------------- grid stencil
+// A segment takes the stencil path when it holds one cloth that is an R x C vertex grid with
+// the triangulation of GenerateClothMesh (Velvet/Scene.hpp:153-165), every stretch constraint
+// joins grid neighbours along a row, a column or a quad diagonal (at most once per pair) and
+// every bending constraint is the dihedral of an interior edge with its two opposite vertices
+// (the per-quad form of Velvet/VtClothObjectGPU.hpp:121-130 or any interior edge). The
+// constraint set and its values are kept exactly; only the storage becomes a stencil.
+struct GridLayout {
+    uint32_t rows = 0, cols = 0;
+    std::vector<float4> rest4;     // per vertex: R, D, X-diagonal, anti-diagonal rest (NaN = none)
+    std::vector<uint32_t> flags;   // per cell: VTX_BX | VTX_BR | VTX_BD
+    std::vector<float4> brest4;    // per cell: X, R, D rest angles
+    bool rest0 = true;
+};
+
+static bool detect_grid(const Segment& sg, GridLayout& G) {
+    if (sg.cloth_starts.size() != 1) return false;
+    const uint32_t n = sg.n();
+    const size_t T = sg.tris.size() / 3;
+    if (n < 4 || T < 2) return false;
+    const uint32_t cols = sg.tris[1];
+    if (cols < 2 || n % cols) return false;
+    const uint32_t rows = n / cols;
+    if (rows < 2 || T != 2ull * (rows - 1) * (cols - 1)) return false;
+    size_t t = 0;
+    for (uint32_t a = 0; a + 1 < rows; a++)
+        for (uint32_t b = 0; b + 1 < cols; b++) {
+            const uint32_t A = a * cols + b, B = A + cols, C = A + 1, D = B + 1;
+            const uint32_t* q = &sg.tris[6 * t / 2];
+            if (q[0] != A || q[1] != B || q[2] != C || q[3] != C || q[4] != B || q[5] != D) return false;
+            t += 2;
+        }
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    G.rows = rows;
+    G.cols = cols;
+    G.rest4.assign(n, make_float4(qnan, qnan, qnan, qnan));
+    const size_t S = sg.st_rest.size();
+    for (size_t c = 0; c < S; c++) {
+        const int64_t i = sg.st[2 * c], j = sg.st[2 * c + 1];
+        const int64_t ai = i / cols, bi = i % cols, aj = j / cols, bj = j % cols;
+        const int64_t da = aj - ai, db = bj - bi;
+        int64_t owner;
+        int comp;
+        if (da == 0 && db == 1) { owner = i; comp = 0; }
+        else if (da == 0 && db == -1) { owner = j; comp = 0; }
+        else if (da == 1 && db == 0) { owner = i; comp = 1; }
+        else if (da == -1 && db == 0) { owner = j; comp = 1; }
+        else if (da == 1 && db == 1) { owner = i; comp = 2; }
+        else if (da == -1 && db == -1) { owner = j; comp = 2; }
+        else if (da == 1 && db == -1) { owner = ai * cols + (bi - 1); comp = 3; }   // (a,b+1)-(a+1,b) owned by (a,b)
+        else if (da == -1 && db == 1) { owner = aj * cols + (bj - 1); comp = 3; }
+        else return false;
+        float* slot = &G.rest4[(size_t)owner].x + comp;
+        if (*slot == *slot) return false;   // a second constraint on the same pair
+        *slot = sg.st_rest[c];
+    }
+    G.flags.assign(n, 0u);
+    G.brest4.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
+    auto rc = [cols](uint32_t v, int64_t& a, int64_t& b) { a = v / cols; b = v % cols; };
+    auto same_pair = [](int64_t x0, int64_t x1, int64_t y0, int64_t y1) {
+        return (x0 == y0 && x1 == y1) || (x0 == y1 && x1 == y0);
+    };
+    const size_t B = sg.bd_rest.size();
+    for (size_t c = 0; c < B; c++) {
+        const uint32_t* q = &sg.bd[4 * c];
+        int64_t au, bu, av, bv;
+        rc(q[2], au, bu);
+        rc(q[3], av, bv);
+        int64_t a, b, o0, o1;
+        int type;
+        if (au == av && (bu - bv == 1 || bv - bu == 1)) {
+            type = 1; a = au; b = std::min(bu, bv);
+            if (a < 1 || a + 1 >= rows) return false;
+            o0 = (a + 1) * cols + b; o1 = (a - 1) * cols + b + 1;
+        } else if (bu == bv && (au - av == 1 || av - au == 1)) {
+            type = 2; a = std::min(au, av); b = bu;
+            if (b < 1 || b + 1 >= cols) return false;
+            o0 = a * cols + b + 1; o1 = (a + 1) * cols + b - 1;
+        } else if ((au - av == 1 && bv - bu == 1) || (av - au == 1 && bu - bv == 1)) {
+            type = 0; a = std::min(au, av); b = std::min(bu, bv);
+            o0 = a * cols + b; o1 = (a + 1) * cols + b + 1;
+        } else {
+            return false;
+        }
+        if (!same_pair(q[0], q[1], o0, o1)) return false;
+        uint32_t& f = G.flags[(size_t)(a * cols + b)];
+        if (f & (1u << type)) return false;
+        f |= 1u << type;
+        (&G.brest4[(size_t)(a * cols + b)].x)[type] = sg.bd_rest[c];
+        if (sg.bd_rest[c] != 0.f) G.rest0 = false;
+    }
+    return true;
+}
+
+void Engine::set_stencil_path(bool on) {
+    if (on == use_stencil_) return;
+    pull_state();
+    use_stencil_ = on;
+    mark_dirty();
+}
+
+int32_t Engine::stencil_segments() {
+    ensure_committed();
+    int32_t k = 0;
+    for (uint8_t g : seg_grid_) k += g;
+    return k;
 }
 
 void Engine::commit() {
@@ -363,6 +490,11 @@ void Engine::commit() {
     std::vector<float4> hslots;
     std::vector<ColliderDev> hcols;
     std::vector<SortTileDev> hst;
+    std::vector<TileDev> hitile;
+    std::vector<GridTileDev> hgtile;
+    std::vector<float4> hrest4, hbrest4;
+    std::vector<uint32_t> hgflags;
+    seg_grid_.assign(segs_.size(), 0);
 
     uint32_t pbase = 0, tbase = 0, sbase = 0, cbase = 0, bbase = 0, trbase = 0, hbase = 0;
     uint64_t nbase = 0;
@@ -405,6 +537,26 @@ void Engine::commit() {
         d.gx = p.gravity[0];
         d.gy = p.gravity[1];
         d.gz = p.gravity[2];
+        if (selfcol) {
+            const uint64_t c = UINT64_MAX / d.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
+            d.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
+            d.mod_hi = (uint32_t)(c >> 32);
+        }
+        GridLayout G;
+        const bool grid = use_stencil_ && detect_grid(sg, G);
+        seg_grid_[si] = grid ? 1 : 0;
+        if (grid) {
+            d.flags |= VTX_SEG_GRID | (G.rest0 ? VTX_SEG_BEND_REST0 : 0u);
+            d.rows = G.rows;
+            d.cols = G.cols;
+            hrest4.insert(hrest4.end(), G.rest4.begin(), G.rest4.end());
+            hgflags.insert(hgflags.end(), G.flags.begin(), G.flags.end());
+            hbrest4.insert(hbrest4.end(), G.brest4.begin(), G.brest4.end());
+        } else {
+            hrest4.resize(hrest4.size() + n, make_float4(0.f, 0.f, 0.f, 0.f));
+            hgflags.resize(hgflags.size() + n, 0u);
+            hbrest4.resize(hbrest4.size() + n, make_float4(0.f, 0.f, 0.f, 0.f));
+        }
         hseg_.push_back(d);
         if (selfcol) {
             any_selfcol_ = true;
@@ -479,11 +631,30 @@ void Engine::commit() {
             t.seg = (uint32_t)si;
             t.begin = pbase + first;
             t.count = std::min<uint32_t>(VTX_TILE, n - first);
-            pack_tile(est, s_st, l_st, first, t.count, make_uint2(VTX_INVALID, 0u), &t.st_off, &t.st_k);
-            pack_tile(eat, s_at, l_at, first, t.count, make_uint2(VTX_INVALID, 0u), &t.at_off, &t.at_k);
-            pack_tile(ebd, s_bd, l_bd, first, t.count, VTX_INVALID, &t.bd_off, &t.bd_k);
+            if (!grid) {   // the stencil path keeps its own attach lists and needs no stretch/bend lists
+                pack_tile(est, s_st, l_st, first, t.count, make_uint2(VTX_INVALID, 0u), &t.st_off, &t.st_k);
+                pack_tile(eat, s_at, l_at, first, t.count, make_uint2(VTX_INVALID, 0u), &t.at_off, &t.at_k);
+                pack_tile(ebd, s_bd, l_bd, first, t.count, VTX_INVALID, &t.bd_off, &t.bd_k);
+            }
             pack_tile(etr, s_tr, l_tr, first, t.count, VTX_INVALID, &t.tr_off, &t.tr_k);
             htile_.push_back(t);
+            if (!grid) hitile.push_back(t);
+        }
+        if (grid) {
+            for (uint32_t a0 = 0; a0 < G.rows; a0 += VTX_GT)
+                for (uint32_t b0 = 0; b0 < G.cols; b0 += VTX_GT) {
+                    GridTileDev gt{};
+                    gt.seg = (uint32_t)si;
+                    gt.a0 = a0;
+                    gt.b0 = b0;
+                    std::vector<int64_t> lanes(VTX_TILE, -1);
+                    for (uint32_t l = 0; l < VTX_TILE; l++) {
+                        const uint32_t a = a0 + l / VTX_GT, b = b0 + l % VTX_GT;
+                        if (a < G.rows && b < G.cols) lanes[l] = (int64_t)a * G.cols + b;
+                    }
+                    pack_lanes(eat, s_at, l_at, lanes, make_uint2(VTX_INVALID, 0u), &gt.at_off, &gt.at_k);
+                    hgtile.push_back(gt);
+                }
         }
         const uint32_t nst = (n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
         for (uint32_t k = 0; k < nst; k++) {
@@ -510,6 +681,13 @@ void Engine::commit() {
     DevState d{};
     d.nparticles = pbase;
     d.ntiles = (uint32_t)htile_.size();
+    d.nitiles = (uint32_t)hitile.size();
+    d.ngtiles = (uint32_t)hgtile.size();
+    d.itiles = dupload(hitile);
+    d.gtiles = dupload(hgtile);
+    d.rest4 = dupload(hrest4);
+    d.gflags = dupload(hgflags);
+    d.brest4 = dupload(hbrest4);
     d.nstiles = (uint32_t)hst.size();
     d.nhist = hbase;
     d.ncells = tbase;
@@ -539,10 +717,10 @@ void Engine::commit() {
         }
         d.hist = dalloc<uint32_t>(hbase);
         d.part = dalloc<uint32_t>((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1);
-        d.cell_start = dalloc<uint32_t>(tbase);
-        d.cell_end = dalloc<uint32_t>(tbase);
-        VTX_HIP(hipMemset(d.cell_start, 0xFF, sizeof(uint32_t) * std::max<uint32_t>(tbase, 1)));
-        VTX_HIP(hipMemset(d.cell_end, 0xFF, sizeof(uint32_t) * std::max<uint32_t>(tbase, 1)));
+        d.cells = dalloc<uint2>(tbase);
+        VTX_HIP(hipMemset(d.cells, 0xFF, sizeof(uint2) * std::max<uint32_t>(tbase, 1)));
+        d.sP = dalloc<float4>(pbase);
+        d.sX0 = dalloc<float4>(pbase);
         d.nbr = dalloc<uint32_t>((size_t)nbase);
         VTX_HIP(hipMemset(d.nbr, 0xFF, sizeof(uint32_t) * std::max<uint64_t>(nbase, 1)));
     }
@@ -589,11 +767,11 @@ void Engine::hash_build(int in, hipStream_t s) {
     }
     stage_end(s);
     stage_begin("hash_cells", s);
-    VTX_HIP(launch_fill_u32(ds_.cell_start, ds_.ncells, VTX_INVALID, s));
-    VTX_HIP(launch_cell_bounds(ds_, src, s));
+    VTX_HIP(launch_fill_u32((uint32_t*)ds_.cells, 2 * ds_.ncells, VTX_INVALID, s));
+    VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, in, src, s));
+    VTX_HIP(launch_neighbors(ds_, src, s));
     stage_end(s);
     sorted_src_ = src;
 }
@@ -825,8 +1003,14 @@ void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint
     synchronize();
     const SegDev& d = hseg_[(size_t)s];
     VTX_REQUIRE(d.flags & VTX_SEG_SELFCOL, VTX_ESTATE, "segment has no spatial hash");
-    if (cs) VTX_HIP(hipMemcpy(cs, ds_.cell_start + d.tbase, sizeof(uint32_t) * d.M, hipMemcpyDeviceToHost));
-    if (ce) VTX_HIP(hipMemcpy(ce, ds_.cell_end + d.tbase, sizeof(uint32_t) * d.M, hipMemcpyDeviceToHost));
+    if (cs || ce) {
+        std::vector<uint2> tmp(d.M);
+        VTX_HIP(hipMemcpy(tmp.data(), ds_.cells + d.tbase, sizeof(uint2) * d.M, hipMemcpyDeviceToHost));
+        for (uint32_t h = 0; h < d.M; h++) {
+            if (cs) cs[h] = tmp[h].x;
+            if (ce) ce[h] = tmp[h].y;
+        }
+    }
     if (sh) VTX_HIP(hipMemcpy(sh, ds_.keys[sorted_src_] + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
     if (si) VTX_HIP(hipMemcpy(si, ds_.vals[sorted_src_] + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
     if (nbr) {

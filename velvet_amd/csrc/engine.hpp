@@ -43,6 +43,7 @@ struct Segment {
     std::vector<float> at_dist;              // A
     std::vector<float> slots;                // 3L
     std::vector<vtx_collider> cols;
+    std::vector<uint32_t> cloth_starts;      // first particle of each AddCloth
     uint32_t n() const { return (uint32_t)w.size(); }
     uint32_t table_size() const {
         return prm.hash_table_size > 0 ? (uint32_t)prm.hash_table_size : 2u * n();
@@ -78,6 +79,8 @@ public:
     void simulate(float frame_dt);
     void synchronize();
     void set_use_graph(bool on);
+    void set_stencil_path(bool on);
+    int32_t stencil_segments();
     void set_stream(hipStream_t s);
     void set_timing(bool on);
     std::string timing_report();
@@ -130,6 +133,8 @@ private:
     int substeps_ = 1, iterations_ = 0, interleave_ = 1;
 
     bool use_graph_ = true;
+    bool use_stencil_ = true;
+    std::vector<uint8_t> seg_grid_;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;
 

@@ -17,13 +17,19 @@
 
 namespace vtx {
 
-// HashCoords (Velvet/SpatialHashGPU.cu:18-22): 32-bit wrap-around products, C remainder.
-__device__ __forceinline__ uint32_t hash_coords(int32_t x, int32_t y, int32_t z, int32_t M) {
+// HashCoords (Velvet/SpatialHashGPU.cu:18-22): 32-bit wrap-around products, then
+// abs(h % M). |h % M| == |h| mod M for C's truncating remainder, and the unsigned remainder is
+// taken by multiplication with c = ceil(2^64 / M): ((c * a) mod 2^64) * M >> 64 == a mod M for
+// all 32-bit a and M (Lemire, Kaser, Kurz, "Faster remainder by direct computation", 2019).
+__device__ __forceinline__ uint32_t hash_coords(int32_t x, int32_t y, int32_t z, uint32_t M, uint64_t c) {
     const int32_t h = (int32_t)(((uint32_t)x * 92837111u) ^ ((uint32_t)y * 689287499u) ^
                                 ((uint32_t)z * 283923481u));
-    const int32_t r = h % M;
-    return (uint32_t)(r < 0 ? -r : r);
+    const uint32_t a = h < 0 ? (uint32_t)(-(int64_t)h) : (uint32_t)h;
+    const uint64_t low = c * (uint64_t)a;
+    return (uint32_t)__umul64hi(low, (uint64_t)M);
 }
+
+__device__ __forceinline__ uint64_t seg_mod(const SegDev& sg) { return (uint64_t)sg.mod_hi << 32 | sg.mod_lo; }
 
 // ComputeIntCoord (Velvet/SpatialHashGPU.cu:13-16): floor of a correctly rounded binary32
 // quotient (the library is built with -fhip-fp32-correctly-rounded-divide-sqrt).
@@ -44,7 +50,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
     }
     const float4 p = d.P[in][g];
     d.keys[0][g] = hash_coords(int_coord(p.x, sg.cell), int_coord(p.y, sg.cell),
-                               int_coord(p.z, sg.cell), (int32_t)sg.M);
+                               int_coord(p.z, sg.cell), sg.M, seg_mod(sg));
 }
 
 // Per-tile digit histogram into hist[hbase + digit * ntiles + tidx].
@@ -190,9 +196,10 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
 }
 
-// FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) per segment, with local sorted
-// positions; empty cells keep the 0xffffffff written by the fill.
-__global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int src) {
+// FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) per segment with local sorted
+// positions (cells were filled with ~0u: empty), plus the gather of predicted and initial
+// positions into sorted order for the neighbour search.
+__global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, int src) {
     const TileDev t = d.tiles[blockIdx.x];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -201,58 +208,61 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int src) {
     const uint32_t local = g - sg.pbase;
     const uint32_t* __restrict__ sk = d.keys[src];
     const uint32_t key = sk[g];
+    uint2* __restrict__ cells = d.cells + sg.tbase;
     if (local == 0) {
-        d.cell_start[sg.tbase + key] = 0;
+        cells[key].x = 0;
     } else {
         const uint32_t prev = sk[g - 1];
         if (key != prev) {
-            d.cell_start[sg.tbase + key] = local;
-            d.cell_end[sg.tbase + prev] = local;
+            cells[key].x = local;
+            cells[prev].y = local;
         }
     }
-    if (local == sg.n - 1) d.cell_end[sg.tbase + key] = sg.n;
+    if (local == sg.n - 1) cells[key].y = sg.n;
+    const uint32_t id = d.vals[src][g];
+    d.sP[g] = d.P[in][sg.pbase + id];
+    d.sX0[g] = d.X0[sg.pbase + id];
 }
 
-// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130), one thread per particle in index
-// order (so the column-major table writes coalesce). Cells are visited x-, y-, z-nested;
-// inside a cell in sorted order, at most max_nbr entries per cell and max_nbr in total; the
-// list ends with 0xffffffff unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
-__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
+// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130): one thread per particle in sorted
+// order, as the reference launches it, so the lanes of a wave probe the same cells; candidate
+// positions are read from the sorted copies (contiguous per cell). Cells are visited x-, y-,
+// z-nested; inside a cell in sorted order, at most max_nbr entries per cell and in total; the
+// list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
+__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int src) {
 #pragma clang fp contract(off)
     const TileDev t = d.tiles[blockIdx.x];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
-    const uint32_t g = t.begin + threadIdx.x;
-    const uint32_t id = g - sg.pbase;
+    const uint32_t g = t.begin + threadIdx.x;        // sorted position
     const uint32_t n = sg.n;
-    const float4* __restrict__ P = d.P[in];
-    const float4* __restrict__ X0 = d.X0;
     const uint32_t* __restrict__ sv = d.vals[src] + sg.pbase;
-    const uint32_t* __restrict__ cs = d.cell_start + sg.tbase;
-    const uint32_t* __restrict__ ce = d.cell_end + sg.tbase;
+    const float4* __restrict__ sP = d.sP + sg.pbase;
+    const float4* __restrict__ sX0 = d.sX0 + sg.pbase;
+    const uint2* __restrict__ cells = d.cells + sg.tbase;
+    const uint32_t id = d.vals[src][g];
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    const float4 p = P[g];
-    const float4 o = X0[g];
+    const float4 p = d.sP[g];
+    const float4 o = d.sX0[g];
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
-    const int32_t M = (int32_t)sg.M;
+    const uint64_t mc = seg_mod(sg);
     const uint32_t K = sg.max_nbr;
     uint32_t k = 0;
     for (int32_t xx = ix - 1; xx <= ix + 1; xx++)
         for (int32_t yy = iy - 1; yy <= iy + 1; yy++)
             for (int32_t zz = iz - 1; zz <= iz + 1; zz++) {
-                const uint32_t h = hash_coords(xx, yy, zz, M);
-                const uint32_t start = cs[h];
-                if (start == VTX_INVALID) continue;
-                const uint32_t end = min(ce[h], start + K);
-                for (uint32_t s = start; s < end; s++) {
+                const uint2 c = cells[hash_coords(xx, yy, zz, sg.M, mc)];
+                if (c.x == VTX_INVALID) continue;
+                const uint32_t end = min(c.y, c.x + K);
+                for (uint32_t s = c.x; s < end; s++) {
                     const uint32_t j = sv[s];
                     if (j == id) continue;
-                    const float4 q = P[sg.pbase + j];
+                    const float4 q = sP[s];
                     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     if (!(d2 < sg.cell2)) continue;
-                    const float4 r = X0[sg.pbase + j];
+                    const float4 r = sX0[s];
                     const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
                     const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
                     if (!(e2 > sg.diam2)) continue;
@@ -295,13 +305,13 @@ hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_cell_bounds(const DevState& d, int src, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_bounds, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, src);
+hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_bounds, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
+hipError_t launch_neighbors(const DevState& d, int src, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, src);
     return hipGetLastError();
 }
 

@@ -164,98 +164,75 @@ __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float 
 // Epilogue modes of the Jacobi pass.
 enum { EPI_NONE = 0, EPI_FINALIZE = 1, EPI_FINALIZE_PREDICT = 2 };
 
-// One Jacobi pass: SolveStretch (cu:65-102) + SolveAttachment (cu:205-235) +
-// SolveBending (cu:117-189) gathered per particle, then ApplyDeltas (cu:253-264); with an
-// epilogue, Finalize (cu:388-407) and optionally the next substep's Predict (cu:42-53).
-template <int EPI, bool CONSTRAINTS>
-__global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, int in, float h) {
-    const TileDev t = d.tiles[blockIdx.x];
-    const uint32_t lane = threadIdx.x;
-    if (lane >= t.count) return;
-    const SegDev sg = d.segs[t.seg];
-    const uint32_t g = t.begin + lane;
-    const float4* __restrict__ P = d.P[in];
-    const float4 p4 = P[g];
-    V3 p = xyz(p4);
-    const float w = p4.w;
+// Hardware reciprocal / reciprocal square root / square root (1 ulp) for the Jacobi pass.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-    if (CONSTRAINTS) {
-        V3 delta = v3(0.f, 0.f, 0.f);
-        int cnt = 0;
-        // stretch: bilateral distance constraint, compliance 0 (cu:85-98)
-        const uint2* __restrict__ st = d.st + t.st_off + lane;
-        for (uint32_t k = 0; k < t.st_k; k++) {
-            const uint2 e = st[(size_t)k * VTX_TILE];
-            if (e.x == VTX_INVALID) continue;
-            const float4 pj4 = P[e.x];
-            const float rest = __uint_as_float(e.y);
-            const V3 diff = p - xyz(pj4);
-            const float dist = length(diff);
-            const float wsum = w + pj4.w;
-            if (dist != rest && wsum > 0.f) {
-                const V3 grad = diff / (dist + VTX_EPSILON);
-                const float lambda = (dist - rest) / wsum * sg.stiffness;
-                delta += -w * (lambda * grad);
-                cnt++;
-            }
-        }
-        // long-range attachment: one-sided, pinned particles skip non-zero targets (cu:218-234)
-        const uint2* __restrict__ at = d.at + t.at_off + lane;
-        for (uint32_t k = 0; k < t.at_k; k++) {
-            const uint2 e = at[(size_t)k * VTX_TILE];
-            if (e.x == VTX_INVALID) continue;
-            const float target = __uint_as_float(e.y) * sg.lrs;
-            if (w == 0.f && target > 0.f) continue;
-            const V3 diff = p - xyz(d.slots[e.x]);
-            const float dist = length(diff);
-            if (dist > target) {
-                delta += -diff + diff / dist * target;
-                cnt++;
-            }
-        }
-        // dihedral bending (cu:128-188); this particle takes the correction of its role
-        const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
-        const float xpbd_bend = sg.bend_compliance / h / h;
-        for (uint32_t k = 0; k < t.bd_k; k++) {
-            const uint32_t e = bd[(size_t)k * VTX_TILE];
-            if (e == VTX_INVALID) continue;
-            const uint32_t bid = e & 0x3FFFFFFFu;
-            const uint32_t role = e >> 30;
-            const uint4 ix = d.brec[bid];
-            const float rest = d.brest[bid];
-            const float4 q0 = P[ix.x], q1 = P[ix.y], q2 = P[ix.z], q3 = P[ix.w];
-            const V3 p0 = xyz(q0), p1 = xyz(q1), p2 = xyz(q2), p3 = xyz(q3);
-            const V3 e3 = p3 - p2;
-            const float elen = length(e3);
-            if (elen < VTX_EPSILON) continue;
-            const float inv_elen = 1.0f / elen;
-            V3 n1 = cross(p2 - p0, p3 - p0);
-            n1 = n1 / dot(n1, n1);
-            V3 n2 = cross(p3 - p1, p2 - p1);
-            n2 = n2 / dot(n2, n2);
-            const V3 d0 = elen * n1;
-            const V3 d1 = elen * n2;
-            const V3 d2 = dot(p0 - p3, e3) * inv_elen * n1 + dot(p1 - p3, e3) * inv_elen * n2;
-            const V3 d3 = dot(p2 - p0, e3) * inv_elen * n1 + dot(p2 - p1, e3) * inv_elen * n2;
-            n1 = normalize(n1);
-            n2 = normalize(n2);
-            const float cd = fminf(fmaxf(dot(n1, n2), -1.0f), 1.0f);
-            const float phi = acosf(cd);
-            float lambda = q0.w * dot(d0, d0) + q1.w * dot(d1, d1) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
-            if (lambda < VTX_EPSILON) continue;
-            lambda = (phi - rest) / (lambda + xpbd_bend);
-            if (dot(cross(n1, n2), e3) > 0.0f) lambda = -lambda;
-            const V3 dk = role == 0 ? d0 : role == 1 ? d1 : role == 2 ? d2 : d3;
-            delta += -w * lambda * dk;
-            cnt++;
-        }
-        if (cnt > 0) p += delta / (float)cnt * sg.omega;
+// One stretch constraint seen from one endpoint (Velvet/VtClothSolverGPU.cu:80-98): bilateral,
+// compliance 0, eps in the gradient denominator. q = other endpoint (xyz, inverse mass).
+__device__ __forceinline__ void stretch_term(V3 p, float w, float4 q, float rest, float stiff, V3& delta,
+                                             int& cnt) {
+    const V3 diff = p - xyz(q);
+    const float dist = fsqrt(dot(diff, diff));
+    const float wsum = w + q.w;
+    if (dist != rest && wsum > 0.f) {
+        const float s = (dist - rest) * frcp(wsum) * stiff * frcp(dist + VTX_EPSILON);
+        delta += (-w * s) * diff;
+        cnt++;
     }
+}
 
+// One long-range attachment (Velvet/VtClothSolverGPU.cu:218-234).
+__device__ __forceinline__ void attach_term(V3 p, float w, float4 slot, float dist0, float lrs, V3& delta,
+                                            int& cnt) {
+    const float target = dist0 * lrs;
+    if (w == 0.f && target > 0.f) return;
+    const V3 diff = p - xyz(slot);
+    const float dist = fsqrt(dot(diff, diff));
+    if (dist > target) {
+        delta += -diff + diff * (target * frcp(dist));
+        cnt++;
+    }
+}
+
+// The dihedral bending constraint (Velvet/VtClothSolverGPU.cu:128-177). With c1, c2 the raw
+// triangle normals, the reference's n1 = c1 / |c1|^2 and normalize(n1) = c1 / |c1| are formed
+// from one reciprocal square root each. Returns false on the reference's early-outs; else
+// lam carries the sign rule and the correction of vertex k is -w_k * lam * d_k.
+__device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float4 q3, float rest, float xpbd,
+                                          float& lam, V3& d0, V3& d1, V3& d2, V3& d3) {
+    const V3 p0 = xyz(q0), p1 = xyz(q1), p2 = xyz(q2), p3 = xyz(q3);
+    const V3 e = p3 - p2;
+    const float elen = fsqrt(dot(e, e));
+    if (elen < VTX_EPSILON) return false;
+    const float inv_elen = frcp(elen);
+    const V3 c1 = cross(p2 - p0, p3 - p0);
+    const V3 c2 = cross(p3 - p1, p2 - p1);
+    const float r1 = frsq(dot(c1, c1)), r2 = frsq(dot(c2, c2));
+    const V3 u1 = c1 * r1, u2 = c2 * r2;
+    const V3 n1 = u1 * r1, n2 = u2 * r2;
+    d0 = elen * n1;
+    d1 = elen * n2;
+    d2 = (dot(p0 - p3, e) * inv_elen) * n1 + (dot(p1 - p3, e) * inv_elen) * n2;
+    d3 = (dot(p2 - p0, e) * inv_elen) * n1 + (dot(p2 - p1, e) * inv_elen) * n2;
+    const float cd = fminf(fmaxf(dot(u1, u2), -1.0f), 1.0f);
+    const float phi = acosf(cd);
+    float l = q0.w * dot(d0, d0) + q1.w * dot(d1, d1) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
+    if (l < VTX_EPSILON) return false;
+    l = (phi - rest) * frcp(l + xpbd);
+    if (dot(cross(u1, u2), e) > 0.0f) l = -l;
+    lam = l;
+    return true;
+}
+
+// Finalize (Velvet/VtClothSolverGPU.cu:396-406) and optionally the next substep's Predict.
+template <int EPI>
+__device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, uint32_t g, int in, V3 p, float w,
+                                         float h) {
     if (EPI == EPI_NONE) {
         d.P[in ^ 1][g] = f4(p, w);
     } else {
-        // Finalize (cu:396-406)
         const float4 x4 = d.X[g];
         const V3 x = xyz(x4);
         V3 raw = (p - x) / h;
@@ -272,6 +249,201 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, int in, float 
         }
         d.V[g] = f4(v, 0.f);
     }
+}
+
+// One Jacobi pass on the general path: SolveStretch (cu:65-102) + SolveAttachment
+// (cu:205-235) + SolveBending (cu:117-189) gathered per particle over its incident
+// constraints (sliced ELL), then ApplyDeltas (cu:253-264) and the epilogue.
+template <int EPI, bool CONSTRAINTS>
+__global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev* __restrict__ tiles, int in,
+                                                      float h) {
+    const TileDev t = tiles[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    if (lane >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    const uint32_t g = t.begin + lane;
+    const float4* __restrict__ P = d.P[in];
+    const float4 p4 = P[g];
+    V3 p = xyz(p4);
+    const float w = p4.w;
+
+    if (CONSTRAINTS) {
+        V3 delta = v3(0.f, 0.f, 0.f);
+        int cnt = 0;
+        const uint2* __restrict__ st = d.st + t.st_off + lane;
+        for (uint32_t k = 0; k < t.st_k; k++) {
+            const uint2 e = st[(size_t)k * VTX_TILE];
+            if (e.x == VTX_INVALID) continue;
+            stretch_term(p, w, P[e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
+        }
+        const uint2* __restrict__ at = d.at + t.at_off + lane;
+        for (uint32_t k = 0; k < t.at_k; k++) {
+            const uint2 e = at[(size_t)k * VTX_TILE];
+            if (e.x == VTX_INVALID) continue;
+            attach_term(p, w, d.slots[e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+        }
+        const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
+        const float xpbd_bend = sg.bend_compliance / h / h;   // cu:173
+        for (uint32_t k = 0; k < t.bd_k; k++) {
+            const uint32_t e = bd[(size_t)k * VTX_TILE];
+            if (e == VTX_INVALID) continue;
+            const uint32_t bid = e & 0x3FFFFFFFu;
+            const uint32_t role = e >> 30;
+            const uint4 ix = d.brec[bid];
+            float lam;
+            V3 d0, d1, d2, d3;
+            if (!bend_eval(P[ix.x], P[ix.y], P[ix.z], P[ix.w], d.brest[bid], xpbd_bend, lam, d0, d1, d2, d3)) continue;
+            const V3 dk = role == 0 ? d0 : role == 1 ? d1 : role == 2 ? d2 : d3;
+            delta += (-w * lam) * dk;
+            cnt++;
+        }
+        if (cnt > 0) p += delta / (float)cnt * sg.omega;
+    }
+    epilogue<EPI>(d, sg, g, in, p, w, h);
+}
+
+// ---------------------------------------------------------------- grid stencil path
+// A segment whose cloth is an R x C vertex grid with the reference's topology (structural +
+// shear stretch, bending on the X/R/D slots of the cells) runs its Jacobi pass on 16 x 16
+// tiles. The tile's positions plus a halo of 2 are staged in LDS; every bending slot touching
+// the tile is evaluated once (not once per vertex), its four corrections are kept in LDS, and
+// each vertex gathers its 8 stretch, its attachments and its 12 bending terms in a fixed order.
+// Slot owner ranges relative to the tile origin (a0, b0): X rows/cols [-1, 15] (17 x 17),
+// R rows [-1, 16] cols [-1, 15] (18 x 17), D rows [-1, 15] cols [-1, 16] (17 x 18).
+#define VTX_NSX (17 * 17)
+#define VTX_NSR (18 * 17)
+#define VTX_NSD (17 * 18)
+#define VTX_NSLOT (VTX_NSX + VTX_NSR + VTX_NSD)
+
+__device__ __forceinline__ int slot_x(int ra, int rb) { return (ra + 1) * 17 + (rb + 1); }
+__device__ __forceinline__ int slot_r(int ra, int rb) { return VTX_NSX + (ra + 1) * 17 + (rb + 1); }
+__device__ __forceinline__ int slot_d(int ra, int rb) { return VTX_NSX + VTX_NSR + (ra + 1) * 18 + (rb + 1); }
+
+__device__ __forceinline__ float4 nan4() {
+    const float q = __int_as_float(0x7FC00000);
+    return make_float4(q, q, q, q);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, float h) {
+    __shared__ float4 sp[VTX_GP * VTX_GP];
+    __shared__ float corr[12][VTX_NSLOT];          // plane r*3+c: component c of role r
+    __shared__ uint8_t act[VTX_NSLOT];
+    const GridTileDev t = d.gtiles[blockIdx.x];
+    const SegDev sg = d.segs[t.seg];
+    const int rows = (int)sg.rows, cols = (int)sg.cols;
+    const int a0 = (int)t.a0, b0 = (int)t.b0;
+    const uint32_t tid = threadIdx.x;
+    const float4* __restrict__ P = d.P[in];
+
+    // stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
+    for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += VTX_TILE) {
+        const int a = a0 - VTX_GH + (int)(e / VTX_GP), b = b0 - VTX_GH + (int)(e % VTX_GP);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (a >= 0 && a < rows && b >= 0 && b < cols) v = P[sg.pbase + (uint32_t)(a * cols + b)];
+        sp[e] = v;
+    }
+    __syncthreads();
+
+    // evaluate every bending slot whose owner cell lies in the tile's slot ranges
+    const float xpbd_bend = sg.bend_compliance / h / h;
+    const bool rest0 = (sg.flags & VTX_SEG_BEND_REST0) != 0;
+    for (uint32_t j = tid; j < VTX_NSLOT; j += VTX_TILE) {
+        int type, ra, rb;
+        if (j < VTX_NSX) { type = 0; ra = (int)j / 17 - 1; rb = (int)j % 17 - 1; }
+        else if (j < VTX_NSX + VTX_NSR) { type = 1; ra = (int)(j - VTX_NSX) / 17 - 1; rb = (int)(j - VTX_NSX) % 17 - 1; }
+        else { type = 2; ra = (int)(j - VTX_NSX - VTX_NSR) / 18 - 1; rb = (int)(j - VTX_NSX - VTX_NSR) % 18 - 1; }
+        const int a = a0 + ra, b = b0 + rb;
+        bool on = false;
+        if (a >= 0 && a < rows && b >= 0 && b < cols) {
+            const uint32_t cell = sg.pbase + (uint32_t)(a * cols + b);
+            on = (d.gflags[cell] >> type) & 1u;
+            if (on) {
+                // canonical vertex offsets per type (see vtx_device.hpp)
+                const int r = ra + VTX_GH, c = rb + VTX_GH;    // owner in staged coordinates
+                int i0, i1, i2, i3;
+                if (type == 0) {
+                    i0 = r * VTX_GP + c; i1 = (r + 1) * VTX_GP + c + 1; i2 = r * VTX_GP + c + 1; i3 = (r + 1) * VTX_GP + c;
+                } else if (type == 1) {
+                    i0 = (r - 1) * VTX_GP + c + 1; i1 = (r + 1) * VTX_GP + c; i2 = r * VTX_GP + c; i3 = r * VTX_GP + c + 1;
+                } else {
+                    i0 = (r + 1) * VTX_GP + c - 1; i1 = r * VTX_GP + c + 1; i2 = (r + 1) * VTX_GP + c; i3 = r * VTX_GP + c;
+                }
+                const float4 q0 = sp[i0], q1 = sp[i1], q2 = sp[i2], q3 = sp[i3];
+                const float rest = rest0 ? 0.f : (type == 0 ? d.brest4[cell].x : type == 1 ? d.brest4[cell].y : d.brest4[cell].z);
+                float lam;
+                V3 d0, d1, d2, d3;
+                on = bend_eval(q0, q1, q2, q3, rest, xpbd_bend, lam, d0, d1, d2, d3);
+                if (on) {
+                    const V3 c0 = (-q0.w * lam) * d0, c1 = (-q1.w * lam) * d1, c2 = (-q2.w * lam) * d2,
+                             c3 = (-q3.w * lam) * d3;
+                    corr[0][j] = c0.x; corr[1][j] = c0.y; corr[2][j] = c0.z;
+                    corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
+                    corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
+                    corr[9][j] = c3.x; corr[10][j] = c3.y; corr[11][j] = c3.z;
+                }
+            }
+        }
+        act[j] = on ? 1 : 0;
+    }
+    __syncthreads();
+
+    const int ty = (int)(tid / VTX_GT), tx = (int)(tid % VTX_GT);
+    const int a = a0 + ty, b = b0 + tx;
+    if (a >= rows || b >= cols) return;
+    const uint32_t g = sg.pbase + (uint32_t)(a * cols + b);
+    const int sc = (ty + VTX_GH) * VTX_GP + (tx + VTX_GH);    // staged index of this vertex
+    const float4 p4 = sp[sc];
+    V3 p = xyz(p4);
+    const float w = p4.w;
+    V3 delta = v3(0.f, 0.f, 0.f);
+    int cnt = 0;
+
+    // stretch: R own/left, D own/up, X-diagonal own/up-left, anti-diagonal of left/up
+    {
+        const float4 r_own = d.rest4[g];
+        const float4 r_left = b > 0 ? d.rest4[g - 1] : nan4();
+        const float4 r_up = a > 0 ? d.rest4[g - cols] : nan4();
+        const float4 r_ul = (a > 0 && b > 0) ? d.rest4[g - cols - 1] : nan4();
+        const float k = sg.stiffness;
+        if (r_own.x == r_own.x) stretch_term(p, w, sp[sc + 1], r_own.x, k, delta, cnt);
+        if (r_left.x == r_left.x) stretch_term(p, w, sp[sc - 1], r_left.x, k, delta, cnt);
+        if (r_own.y == r_own.y) stretch_term(p, w, sp[sc + VTX_GP], r_own.y, k, delta, cnt);
+        if (r_up.y == r_up.y) stretch_term(p, w, sp[sc - VTX_GP], r_up.y, k, delta, cnt);
+        if (r_own.z == r_own.z) stretch_term(p, w, sp[sc + VTX_GP + 1], r_own.z, k, delta, cnt);
+        if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
+        if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
+        if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
+    }
+    // attachments
+    {
+        const uint2* __restrict__ at = d.at + t.at_off + tid;
+        for (uint32_t k = 0; k < t.at_k; k++) {
+            const uint2 e = at[(size_t)k * VTX_TILE];
+            if (e.x == VTX_INVALID) continue;
+            attach_term(p, w, d.slots[e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+        }
+    }
+    // bending: X roles 0..3, R roles 0..3, D roles 0..3
+    {
+        const int sx[4] = {slot_x(ty, tx), slot_x(ty - 1, tx - 1), slot_x(ty, tx - 1), slot_x(ty - 1, tx)};
+        const int sr[4] = {slot_r(ty + 1, tx - 1), slot_r(ty - 1, tx), slot_r(ty, tx), slot_r(ty, tx - 1)};
+        const int sd[4] = {slot_d(ty - 1, tx + 1), slot_d(ty, tx - 1), slot_d(ty - 1, tx), slot_d(ty, tx)};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (act[sx[r]]) { delta += v3(corr[3 * r][sx[r]], corr[3 * r + 1][sx[r]], corr[3 * r + 2][sx[r]]); cnt++; }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (act[sr[r]]) { delta += v3(corr[3 * r][sr[r]], corr[3 * r + 1][sr[r]], corr[3 * r + 2][sr[r]]); cnt++; }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (act[sd[r]]) { delta += v3(corr[3 * r][sd[r]], corr[3 * r + 1][sd[r]], corr[3 * r + 2][sd[r]]); cnt++; }
+        }
+    }
+    if (cnt > 0) p += delta / (float)cnt * sg.omega;
+    epilogue<EPI>(d, sg, g, in, p, w, h);
 }
 
 // ComputeTriangleNormals + ComputeVertexNormals (Velvet/VtClothSolverGPU.cu:419-450) as a
@@ -310,16 +482,26 @@ hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s) {
 
 hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints,
                           hipStream_t s) {
-    if (!d.ntiles) return hipSuccess;
-    dim3 gr(d.ntiles), bl(VTX_TILE);
-    if (constraints) {
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, true>), gr, bl, 0, s, d, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, true>), gr, bl, 0, s, d, in, h);
-        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, true>), gr, bl, 0, s, d, in, h);
-    } else {
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, false>), gr, bl, 0, s, d, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, false>), gr, bl, 0, s, d, in, h);
-        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, false>), gr, bl, 0, s, d, in, h);
+    dim3 bl(VTX_TILE);
+    if (!constraints) {   // no Jacobi pass: only the epilogue, over every particle
+        if (!d.ntiles) return hipSuccess;
+        dim3 gr(d.ntiles);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, false>), gr, bl, 0, s, d, d.tiles, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, false>), gr, bl, 0, s, d, d.tiles, in, h);
+        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, false>), gr, bl, 0, s, d, d.tiles, in, h);
+        return last_launch();
+    }
+    if (d.ngtiles) {
+        dim3 gg(d.ngtiles);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate_grid<EPI_NONE>), gg, bl, 0, s, d, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE>), gg, bl, 0, s, d, in, h);
+        else hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE_PREDICT>), gg, bl, 0, s, d, in, h);
+    }
+    if (d.nitiles) {
+        dim3 gr(d.nitiles);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, true>), gr, bl, 0, s, d, d.itiles, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, true>), gr, bl, 0, s, d, d.itiles, in, h);
+        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, true>), gr, bl, 0, s, d, d.itiles, in, h);
     }
     return last_launch();
 }

@@ -22,6 +22,21 @@
 #define VTX_MAX_NEIGHBORS 64
 
 #define VTX_SEG_SELFCOL 1u
+#define VTX_SEG_GRID 2u          // Jacobi pass runs on 2-D grid tiles (stencil layout)
+#define VTX_SEG_BEND_REST0 4u    // every bending rest angle is 0 (no rest-angle loads)
+
+// 2-D grid tiles of the stencil path: VTX_GT x VTX_GT particles, halo of 2 rows/cols.
+#define VTX_GT 16
+#define VTX_GH 2
+#define VTX_GP (VTX_GT + 2 * VTX_GH)      // 20: staged positions per side
+// Bending slot types of a grid cell (a, b), each with its canonical vertex order
+// (p0, p1, p2, p3) as (row, col) offsets from the cell:
+//   X: diagonal (a+1,b)-(a,b+1)    (0,0) (1,1) (0,1) (1,0)   -- Velvet/VtClothObjectGPU.hpp:121-130
+//   R: row edge (a,b)-(a,b+1)      (-1,1) (1,0) (0,0) (0,1)
+//   D: column edge (a,b)-(a+1,b)   (1,-1) (0,1) (1,0) (0,0)
+#define VTX_BX 1u
+#define VTX_BR 2u
+#define VTX_BD 4u
 
 namespace vtx {
 
@@ -33,6 +48,8 @@ struct alignas(16) SegDev {
     float max_speed, damping, omega, lrs;
     float bend_compliance, margin, friction, stiffness;
     float gx, gy, gz, pad0;
+    uint32_t mod_lo, mod_hi;               // ceil(2^64 / M): exact remainder by multiplication
+    uint32_t rows, cols;                   // grid segments: vertex grid shape
 };
 
 struct alignas(16) TileDev {
@@ -44,6 +61,11 @@ struct alignas(16) TileDev {
 struct alignas(16) SortTileDev {
     uint32_t seg, begin, count, hbase;     // hbase: histogram base of the segment
     uint32_t ntiles, tidx, pad0, pad1;     // sort tiles of the segment, index of this one
+};
+
+struct alignas(16) GridTileDev {
+    uint32_t seg, a0, b0, pad0;            // segment, first row / column of the tile
+    uint32_t at_off, at_k, pad1, pad2;     // attach ELL of the tile's lanes
 };
 
 // Collider with its transforms as 3x4 row-major affine matrices:
@@ -83,11 +105,17 @@ struct DevState {
     uint32_t* vals[2];        // sort values (local particle index) ping-pong
     uint32_t* hist;           // per (segment, digit, sort tile) counts, scanned in place
     uint32_t* part;           // scan partials
-    uint32_t* cell_start;
-    uint32_t* cell_end;
+    uint2* cells;             // per hash cell: (start, end) in sorted order, start ~0u = empty
+    float4* sP;               // predicted positions gathered into sorted order
+    float4* sX0;              // initial positions gathered into sorted order
     uint32_t* nbr;            // neighbour tables, column-major per segment
+    const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
+    const GridTileDev* gtiles;
+    const float4* rest4;      // grid: stretch rest lengths owned by a vertex (R, D, X-diag, anti-diag)
+    const uint32_t* gflags;   // grid: bending slots present at a cell (VTX_BX | VTX_BR | VTX_BD)
+    const float4* brest4;     // grid: bending rest angles of a cell (X, R, D)
     uint32_t ntiles, nstiles, nhist, ncells;
-    uint32_t nparticles, pad0, pad1, pad2;
+    uint32_t nparticles, nitiles, ngtiles, pad0;
 };
 
 }  // namespace vtx

@@ -65,12 +65,14 @@ def to_vtx_colliders(cols: Sequence[S.Collider], frame_dt=None):
 class ClothSolver:
     """One vtx_solver handle; segments are independent simulations packed together."""
 
-    def __init__(self, params: Optional[S.Params] = None, use_graph: bool = True):
+    def __init__(self, params: Optional[S.Params] = None, use_graph: bool = True, stencil: bool = True):
         h = C.c_void_p()
         vp = to_vtx_params(params or S.Params())
         check(lib().vtx_solver_create(C.byref(vp), C.byref(h)))
         self.h = h
         self.set_use_graph(use_graph)
+        if not stencil:
+            self.set_stencil_path(False)
 
     def close(self):
         if getattr(self, "h", None):
@@ -165,6 +167,14 @@ class ClothSolver:
 
     def set_use_graph(self, on: bool):
         check(lib().vtx_set_use_graph(self.h, int(bool(on))))
+
+    def set_stencil_path(self, on: bool):
+        check(lib().vtx_set_stencil_path(self.h, int(bool(on))))
+
+    def stencil_segments(self) -> int:
+        out = C.c_int32()
+        check(lib().vtx_stencil_segments(self.h, C.byref(out)))
+        return out.value
 
     def set_stream(self, stream_ptr):
         check(lib().vtx_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
