@@ -46,6 +46,21 @@ def hip_stream():
     return s.value
 
 
+def copy_probe(mib):
+    """One device-to-device copy of known size: the calibration kernel for HBM counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def workload_scenes(name):
     if name == "batch512":
         return S.config4(count=512), "512 cloths, sides log-uniform 16..128, 5x10, per-cloth hash grids"
@@ -149,6 +164,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the batched "
@@ -185,6 +202,9 @@ def main():
                 "kernel": "k_iterate", "avg_launch_ms": round(avg_ms, 6),
                 "algorithmic_bytes_per_launch": alg_bytes}
     solver.close()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
 
     plain = None
     if not args.no_plain and len(scenes) > 1:

@@ -547,6 +547,20 @@ void Engine::commit() {
         seg_grid_[si] = grid ? 1 : 0;
         if (grid) {
             d.flags |= VTX_SEG_GRID | (G.rest0 ? VTX_SEG_BEND_REST0 : 0u);
+            // per slot type: all geometrically possible slots present, or none
+            for (int type = 0; type < 3; type++) {
+                bool full = true, none = true;
+                for (uint32_t a = 0; a < G.rows && (full || none); a++)
+                    for (uint32_t b = 0; b < G.cols; b++) {
+                        const bool possible = a + 1 < G.rows && b + 1 < G.cols && (type != 1 || a >= 1) &&
+                                              (type != 2 || b >= 1);
+                        const bool present = (G.flags[(size_t)a * G.cols + b] >> type) & 1u;
+                        if (present != possible) full = false;
+                        if (present) none = false;
+                    }
+                if (full) d.flags |= VTX_SEG_FULL_X << type;
+                if (none) d.flags |= VTX_SEG_NONE_X << type;
+            }
             d.rows = G.rows;
             d.cols = G.cols;
             hrest4.insert(hrest4.end(), G.rest4.begin(), G.rest4.end());
@@ -771,7 +785,7 @@ void Engine::hash_build(int in, hipStream_t s) {
     VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, src, s));
+    VTX_HIP(launch_neighbors(ds_, in, src, s));
     stage_end(s);
     sorted_src_ = src;
 }

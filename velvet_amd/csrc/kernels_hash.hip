@@ -224,53 +224,56 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
     d.sX0[g] = d.X0[sg.pbase + id];
 }
 
-// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130): one thread per particle in sorted
-// order, as the reference launches it, so the lanes of a wave probe the same cells; candidate
-// positions are read from the sorted copies (contiguous per cell). Cells are visited x-, y-,
-// z-nested; inside a cell in sorted order, at most max_nbr entries per cell and in total; the
-// list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
-__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int src) {
+// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130), one thread per particle in index
+// order: neighbouring lanes hold neighbouring cloth vertices, so they probe the same cells and
+// the column-major table writes coalesce. The 27 cell entries are loaded up front (independent
+// loads), candidates come from the sorted copies (contiguous per cell). Cells are visited x-,
+// y-, z-nested; inside a cell in sorted order, at most max_nbr entries per cell and in total;
+// the list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
+__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
 #pragma clang fp contract(off)
     const TileDev t = d.tiles[blockIdx.x];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
-    const uint32_t g = t.begin + threadIdx.x;        // sorted position
+    const uint32_t g = t.begin + threadIdx.x;
+    const uint32_t id = g - sg.pbase;
     const uint32_t n = sg.n;
     const uint32_t* __restrict__ sv = d.vals[src] + sg.pbase;
     const float4* __restrict__ sP = d.sP + sg.pbase;
     const float4* __restrict__ sX0 = d.sX0 + sg.pbase;
     const uint2* __restrict__ cells = d.cells + sg.tbase;
-    const uint32_t id = d.vals[src][g];
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    const float4 p = d.sP[g];
-    const float4 o = d.sX0[g];
+    const float4 p = d.P[in][g];
+    const float4 o = d.X0[g];
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
     const uint64_t mc = seg_mod(sg);
+    uint2 c[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++)
+        c[q] = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
     const uint32_t K = sg.max_nbr;
     uint32_t k = 0;
-    for (int32_t xx = ix - 1; xx <= ix + 1; xx++)
-        for (int32_t yy = iy - 1; yy <= iy + 1; yy++)
-            for (int32_t zz = iz - 1; zz <= iz + 1; zz++) {
-                const uint2 c = cells[hash_coords(xx, yy, zz, sg.M, mc)];
-                if (c.x == VTX_INVALID) continue;
-                const uint32_t end = min(c.y, c.x + K);
-                for (uint32_t s = c.x; s < end; s++) {
-                    const uint32_t j = sv[s];
-                    if (j == id) continue;
-                    const float4 q = sP[s];
-                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (!(d2 < sg.cell2)) continue;
-                    const float4 r = sX0[s];
-                    const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
-                    const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-                    if (!(e2 > sg.diam2)) continue;
-                    nb[(size_t)k * n] = j;
-                    k++;
-                    if (k >= K) return;
-                }
-            }
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        if (c[q].x == VTX_INVALID) continue;
+        const uint32_t end = min(c[q].y, c[q].x + K);
+        for (uint32_t s = c[q].x; s < end; s++) {
+            const uint32_t j = sv[s];
+            if (j == id) continue;
+            const float4 qp = sP[s];
+            const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            if (!(d2 < sg.cell2)) continue;
+            const float4 r = sX0[s];
+            const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
+            const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+            if (!(e2 > sg.diam2)) continue;
+            nb[(size_t)k * n] = j;
+            k++;
+            if (k >= K) return;
+        }
+    }
     nb[(size_t)k * n] = VTX_INVALID;
 }
 
@@ -310,8 +313,8 @@ hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const DevState& d, int src, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, src);
+hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
     return hipGetLastError();
 }
 

@@ -177,7 +177,7 @@ __device__ __forceinline__ void stretch_term(V3 p, float w, float4 q, float rest
     const float dist = fsqrt(dot(diff, diff));
     const float wsum = w + q.w;
     if (dist != rest && wsum > 0.f) {
-        const float s = (dist - rest) * frcp(wsum) * stiff * frcp(dist + VTX_EPSILON);
+        const float s = (dist - rest) * stiff * frcp(wsum * (dist + VTX_EPSILON));
         delta += (-w * s) * diff;
         cnt++;
     }
@@ -215,10 +215,13 @@ __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float
     d0 = elen * n1;
     d1 = elen * n2;
     d2 = (dot(p0 - p3, e) * inv_elen) * n1 + (dot(p1 - p3, e) * inv_elen) * n2;
-    d3 = (dot(p2 - p0, e) * inv_elen) * n1 + (dot(p2 - p1, e) * inv_elen) * n2;
+    // the four gradients of a dihedral angle sum to zero (it is translation invariant):
+    // the reference's d3 = (p2-p0).e/|e| n1 + (p2-p1).e/|e| n2 equals -(d0 + d1 + d2)
+    d3 = -(d0 + d1 + d2);
     const float cd = fminf(fmaxf(dot(u1, u2), -1.0f), 1.0f);
     const float phi = acosf(cd);
-    float l = q0.w * dot(d0, d0) + q1.w * dot(d1, d1) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
+    // |d0|^2 = |e|^2 / |c1|^2 and |d1|^2 = |e|^2 / |c2|^2
+    float l = dot(e, e) * (q0.w * r1 * r1 + q1.w * r2 * r2) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
     if (l < VTX_EPSILON) return false;
     l = (phi - rest) * frcp(l + xpbd);
     if (dot(cross(u1, u2), e) > 0.0f) l = -l;
@@ -357,7 +360,14 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         bool on = false;
         if (a >= 0 && a < rows && b >= 0 && b < cols) {
             const uint32_t cell = sg.pbase + (uint32_t)(a * cols + b);
-            on = (d.gflags[cell] >> type) & 1u;
+            const uint32_t full = (sg.flags >> 3) & 7u, none = (sg.flags >> 6) & 7u;
+            if ((none >> type) & 1u) {
+                on = false;
+            } else if ((full >> type) & 1u) {   // X: a<=R-2,b<=C-2; R: 1<=a<=R-2,b<=C-2; D: a<=R-2,1<=b<=C-2
+                on = a + 1 < rows && b + 1 < cols && (type != 1 || a >= 1) && (type != 2 || b >= 1);
+            } else {
+                on = (d.gflags[cell] >> type) & 1u;
+            }
             if (on) {
                 // canonical vertex offsets per type (see vtx_device.hpp)
                 const int r = ra + VTX_GH, c = rb + VTX_GH;    // owner in staged coordinates

@@ -19,6 +19,6 @@ hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s
 hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
-hipError_t launch_neighbors(const DevState& d, int src, hipStream_t s);
+hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s);
 
 }  // namespace vtx

@@ -24,6 +24,14 @@
 #define VTX_SEG_SELFCOL 1u
 #define VTX_SEG_GRID 2u          // Jacobi pass runs on 2-D grid tiles (stencil layout)
 #define VTX_SEG_BEND_REST0 4u    // every bending rest angle is 0 (no rest-angle loads)
+// Per bending slot type (X, R, D): every geometrically possible slot is present (presence is
+// computed from the cell position) or none is; otherwise presence is read per cell.
+#define VTX_SEG_FULL_X 8u
+#define VTX_SEG_FULL_R 16u
+#define VTX_SEG_FULL_D 32u
+#define VTX_SEG_NONE_X 64u
+#define VTX_SEG_NONE_R 128u
+#define VTX_SEG_NONE_D 256u
 
 // 2-D grid tiles of the stencil path: VTX_GT x VTX_GT particles, halo of 2 rows/cols.
 #define VTX_GT 16
