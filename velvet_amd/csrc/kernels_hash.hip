@@ -39,7 +39,7 @@ __device__ __forceinline__ int32_t int_coord(float v, float cell) {
 
 // ComputeParticleHash_Kernel (Velvet/SpatialHashGPU.cu:34-42) over predicted positions.
 __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -200,7 +200,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
 // positions (cells were filled with ~0u: empty), plus the gather of predicted and initial
 // positions into sorted order for the neighbour search.
 __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, int src) {
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // the list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
 __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
 #pragma clang fp contract(off)
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;

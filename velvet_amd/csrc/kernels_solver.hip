@@ -102,7 +102,7 @@ __device__ V3 collide_sdf(V3 pred, V3 pos, const ColliderDev* cols, uint32_t cb,
 // Pre-stabilisation SDF on positions with the frame time (Velvet/VtClothSolverGPU.hpp:73) and
 // the first substep's PredictPositions (cu:42-53).
 __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float frame_dt, float h) {
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float fram
 // CollideParticles_Kernel + ApplyDeltas (Velvet/VtClothSolverGPU.cu:329-386) followed by the
 // substep's CollideSDF (cu:289-314, hpp:87).
 __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float h) {
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -260,7 +260,7 @@ __device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, ui
 template <int EPI, bool CONSTRAINTS>
 __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev* __restrict__ tiles, int in,
                                                       float h) {
-    const TileDev t = tiles[blockIdx.x];
+    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     __shared__ float4 sp[VTX_GP * VTX_GP];
     __shared__ float corr[12][VTX_NSLOT];          // plane r*3+c: component c of role r
     __shared__ uint8_t act[VTX_NSLOT];
-    const GridTileDev t = d.gtiles[blockIdx.x];
+    const GridTileDev t = d.gtiles[xcd_tile(blockIdx.x, gridDim.x)];
     const SegDev sg = d.segs[t.seg];
     const int rows = (int)sg.rows, cols = (int)sg.cols;
     const int a0 = (int)t.a0, b0 = (int)t.b0;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
 // ComputeTriangleNormals + ComputeVertexNormals (Velvet/VtClothSolverGPU.cu:419-450) as a
 // per-vertex gather over incident triangles.
 __global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d) {
-    const TileDev t = d.tiles[blockIdx.x];
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
     const uint32_t g = t.begin + lane;

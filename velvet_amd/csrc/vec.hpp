@@ -41,4 +41,13 @@ __device__ __forceinline__ V3 xform_vec(const float* m, V3 p) {
               m[8] * p.x + m[9] * p.y + m[10] * p.z};
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (workgroup b runs on XCD
+// b % 8, each XCD with its own L2). Consecutive tiles are spatial neighbours that share halo,
+// cell-table and gather data, so hand each XCD a contiguous run of tiles: block b of nb
+// processes tile xcd_tile(b, nb), a bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
 }  // namespace vtx
