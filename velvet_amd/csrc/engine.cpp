@@ -512,6 +512,8 @@ void Engine::commit() {
         const bool selfcol = p.enable_self_collision && n > 0;
         d.M = selfcol ? sg.table_size() : 0;
         VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
+        VTX_REQUIRE(!selfcol || n < (1u << 25), VTX_ECAPACITY,
+                    "self collision supports at most 2^25 particles per segment");
         d.tbase = tbase;
         d.nbase_lo = (uint32_t)(nbase & 0xFFFFFFFFu);
         d.nbase_hi = (uint32_t)(nbase >> 32);

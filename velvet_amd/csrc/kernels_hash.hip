@@ -232,6 +232,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // the list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
 __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t ranges[27][VTX_TILE];
     const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -248,31 +249,44 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int 
     const float4 o = d.X0[g];
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
     const uint64_t mc = seg_mod(sg);
-    uint2 c[27];
-#pragma unroll
-    for (int q = 0; q < 27; q++)
-        c[q] = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
+    // The 27 cell ranges, x-, y-, z-nested, packed as start << 7 | length (length <= 64,
+    // start < 2^25) in this lane's LDS column; the candidate walk is one flat loop over their
+    // concatenation (same order), so the lanes of a wave diverge only in the total count.
     const uint32_t K = sg.max_nbr;
-    uint32_t k = 0;
+    uint32_t mask = 0, total = 0;
 #pragma unroll
     for (int q = 0; q < 27; q++) {
-        if (c[q].x == VTX_INVALID) continue;
-        const uint32_t end = min(c[q].y, c[q].x + K);
-        for (uint32_t s = c[q].x; s < end; s++) {
-            const uint32_t j = sv[s];
-            if (j == id) continue;
-            const float4 qp = sP[s];
-            const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
-            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            if (!(d2 < sg.cell2)) continue;
-            const float4 r = sX0[s];
-            const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
-            const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-            if (!(e2 > sg.diam2)) continue;
-            nb[(size_t)k * n] = j;
-            k++;
-            if (k >= K) return;
+        const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
+        const uint32_t len = c.x == VTX_INVALID ? 0u : min(c.y, c.x + K) - c.x;
+        ranges[q][threadIdx.x] = c.x << 7 | len;
+        mask |= (len ? 1u : 0u) << q;
+        total += len;
+    }
+    uint32_t k = 0, s = 0, left = 0;
+    for (uint32_t it = 0; it < total; it++) {
+        if (left == 0) {         // next non-empty cell: one exists while it < total
+            const int q = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const uint32_t r = ranges[q][threadIdx.x];
+            s = r >> 7;
+            left = r & 127u;
         }
+        const uint32_t cur = s;
+        s++;
+        left--;
+        const uint32_t j = sv[cur];
+        if (j == id) continue;
+        const float4 qp = sP[cur];
+        const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        if (!(d2 < sg.cell2)) continue;
+        const float4 r = sX0[cur];
+        const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
+        const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+        if (!(e2 > sg.diam2)) continue;
+        nb[(size_t)k * n] = j;
+        k++;
+        if (k >= K) return;
     }
     nb[(size_t)k * n] = VTX_INVALID;
 }

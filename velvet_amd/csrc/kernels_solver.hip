@@ -286,7 +286,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             attach_term(p, w, d.slots[e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
         }
         const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
-        const float xpbd_bend = sg.bend_compliance / h / h;   // cu:173
+        const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
         for (uint32_t k = 0; k < t.bd_k; k++) {
             const uint32_t e = bd[(size_t)k * VTX_TILE];
             if (e == VTX_INVALID) continue;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             delta += (-w * lam) * dk;
             cnt++;
         }
-        if (cnt > 0) p += delta / (float)cnt * sg.omega;
+        if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     }
     epilogue<EPI>(d, sg, g, in, p, w, h);
 }
@@ -321,6 +321,54 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
 __device__ __forceinline__ int slot_x(int ra, int rb) { return (ra + 1) * 17 + (rb + 1); }
 __device__ __forceinline__ int slot_r(int ra, int rb) { return VTX_NSX + (ra + 1) * 17 + (rb + 1); }
 __device__ __forceinline__ int slot_d(int ra, int rb) { return VTX_NSX + VTX_NSR + (ra + 1) * 18 + (rb + 1); }
+
+// Evaluate bending slot j (index within its type's range) of a grid tile into LDS.
+template <int TYPE>
+__device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
+                                          float (*corr)[VTX_NSLOT], uint8_t* act, int a0, int b0, int rows,
+                                          int cols, uint32_t j, float xpbd) {
+    constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
+    constexpr int BASE = TYPE == 0 ? 0 : TYPE == 1 ? VTX_NSX : VTX_NSX + VTX_NSR;
+    const int ra = (int)(j / W) - 1, rb = (int)(j % W) - 1;
+    const int a = a0 + ra, b = b0 + rb;
+    bool on = false;
+    if (a >= 0 && a < rows && b >= 0 && b < cols) {
+        if ((sg.flags >> (6 + TYPE)) & 1u) {
+            on = false;
+        } else if ((sg.flags >> (3 + TYPE)) & 1u) {   // X: a<=R-2,b<=C-2; R: 1<=a<=R-2,b<=C-2; D: a<=R-2,1<=b<=C-2
+            on = a + 1 < rows && b + 1 < cols && (TYPE != 1 || a >= 1) && (TYPE != 2 || b >= 1);
+        } else {
+            on = (d.gflags[sg.pbase + (uint32_t)(a * cols + b)] >> TYPE) & 1u;
+        }
+    }
+    if (on) {
+        const int o = (ra + VTX_GH) * VTX_GP + (rb + VTX_GH);    // owner in staged coordinates
+        // canonical vertex order per type (vtx_device.hpp)
+        constexpr int i0 = TYPE == 0 ? 0 : TYPE == 1 ? -VTX_GP + 1 : VTX_GP - 1;
+        constexpr int i1 = TYPE == 0 ? VTX_GP + 1 : TYPE == 1 ? VTX_GP : 1;
+        constexpr int i2 = TYPE == 0 ? 1 : TYPE == 1 ? 0 : VTX_GP;
+        constexpr int i3 = TYPE == 0 ? VTX_GP : TYPE == 1 ? 1 : 0;
+        const float4 q0 = sp[o + i0], q1 = sp[o + i1], q2 = sp[o + i2], q3 = sp[o + i3];
+        float rest = 0.f;
+        if (!(sg.flags & VTX_SEG_BEND_REST0)) {
+            const float4 r4 = d.brest4[sg.pbase + (uint32_t)(a * cols + b)];
+            rest = TYPE == 0 ? r4.x : TYPE == 1 ? r4.y : r4.z;
+        }
+        float lam;
+        V3 d0, d1, d2, d3;
+        on = bend_eval(q0, q1, q2, q3, rest, xpbd, lam, d0, d1, d2, d3);
+        if (on) {
+            const int s = BASE + (int)j;
+            const V3 c0 = (-q0.w * lam) * d0, c1 = (-q1.w * lam) * d1, c2 = (-q2.w * lam) * d2,
+                     c3 = (-q3.w * lam) * d3;
+            corr[0][s] = c0.x; corr[1][s] = c0.y; corr[2][s] = c0.z;
+            corr[3][s] = c1.x; corr[4][s] = c1.y; corr[5][s] = c1.z;
+            corr[6][s] = c2.x; corr[7][s] = c2.y; corr[8][s] = c2.z;
+            corr[9][s] = c3.x; corr[10][s] = c3.y; corr[11][s] = c3.z;
+        }
+    }
+    act[BASE + j] = on ? 1 : 0;
+}
 
 __device__ __forceinline__ float4 nan4() {
     const float q = __int_as_float(0x7FC00000);
@@ -348,53 +396,18 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     }
     __syncthreads();
 
-    // evaluate every bending slot whose owner cell lies in the tile's slot ranges
-    const float xpbd_bend = sg.bend_compliance / h / h;
-    const bool rest0 = (sg.flags & VTX_SEG_BEND_REST0) != 0;
-    for (uint32_t j = tid; j < VTX_NSLOT; j += VTX_TILE) {
-        int type, ra, rb;
-        if (j < VTX_NSX) { type = 0; ra = (int)j / 17 - 1; rb = (int)j % 17 - 1; }
-        else if (j < VTX_NSX + VTX_NSR) { type = 1; ra = (int)(j - VTX_NSX) / 17 - 1; rb = (int)(j - VTX_NSX) % 17 - 1; }
-        else { type = 2; ra = (int)(j - VTX_NSX - VTX_NSR) / 18 - 1; rb = (int)(j - VTX_NSX - VTX_NSR) % 18 - 1; }
-        const int a = a0 + ra, b = b0 + rb;
-        bool on = false;
-        if (a >= 0 && a < rows && b >= 0 && b < cols) {
-            const uint32_t cell = sg.pbase + (uint32_t)(a * cols + b);
-            const uint32_t full = (sg.flags >> 3) & 7u, none = (sg.flags >> 6) & 7u;
-            if ((none >> type) & 1u) {
-                on = false;
-            } else if ((full >> type) & 1u) {   // X: a<=R-2,b<=C-2; R: 1<=a<=R-2,b<=C-2; D: a<=R-2,1<=b<=C-2
-                on = a + 1 < rows && b + 1 < cols && (type != 1 || a >= 1) && (type != 2 || b >= 1);
-            } else {
-                on = (d.gflags[cell] >> type) & 1u;
-            }
-            if (on) {
-                // canonical vertex offsets per type (see vtx_device.hpp)
-                const int r = ra + VTX_GH, c = rb + VTX_GH;    // owner in staged coordinates
-                int i0, i1, i2, i3;
-                if (type == 0) {
-                    i0 = r * VTX_GP + c; i1 = (r + 1) * VTX_GP + c + 1; i2 = r * VTX_GP + c + 1; i3 = (r + 1) * VTX_GP + c;
-                } else if (type == 1) {
-                    i0 = (r - 1) * VTX_GP + c + 1; i1 = (r + 1) * VTX_GP + c; i2 = r * VTX_GP + c; i3 = r * VTX_GP + c + 1;
-                } else {
-                    i0 = (r + 1) * VTX_GP + c - 1; i1 = r * VTX_GP + c + 1; i2 = (r + 1) * VTX_GP + c; i3 = r * VTX_GP + c;
-                }
-                const float4 q0 = sp[i0], q1 = sp[i1], q2 = sp[i2], q3 = sp[i3];
-                const float rest = rest0 ? 0.f : (type == 0 ? d.brest4[cell].x : type == 1 ? d.brest4[cell].y : d.brest4[cell].z);
-                float lam;
-                V3 d0, d1, d2, d3;
-                on = bend_eval(q0, q1, q2, q3, rest, xpbd_bend, lam, d0, d1, d2, d3);
-                if (on) {
-                    const V3 c0 = (-q0.w * lam) * d0, c1 = (-q1.w * lam) * d1, c2 = (-q2.w * lam) * d2,
-                             c3 = (-q3.w * lam) * d3;
-                    corr[0][j] = c0.x; corr[1][j] = c0.y; corr[2][j] = c0.z;
-                    corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
-                    corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
-                    corr[9][j] = c3.x; corr[10][j] = c3.y; corr[11][j] = c3.z;
-                }
-            }
-        }
-        act[j] = on ? 1 : 0;
+    // evaluate every bending slot whose owner cell lies in the tile's slot ranges: thread t takes
+    // slot t of each type; the remaining 33 X, 50 R and 50 D slots go to waves 0, 1 and 2 so
+    // that every wave evaluates one slot type at a time (no divergent type dispatch)
+    const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);
+    {
+        const uint32_t wave = tid >> 6, lane = tid & 63u;
+        eval_slot<0>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
+        eval_slot<1>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
+        eval_slot<2>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
+        if (wave == 0 && lane < VTX_NSX - VTX_TILE) eval_slot<0>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
+        if (wave == 1 && lane < VTX_NSR - VTX_TILE) eval_slot<1>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
+        if (wave == 2 && lane < VTX_NSD - VTX_TILE) eval_slot<2>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
     }
     __syncthreads();
 
@@ -452,7 +465,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
             if (act[sd[r]]) { delta += v3(corr[3 * r][sd[r]], corr[3 * r + 1][sd[r]], corr[3 * r + 2][sd[r]]); cnt++; }
         }
     }
-    if (cnt > 0) p += delta / (float)cnt * sg.omega;
+    if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);
 }
 
