@@ -91,6 +91,23 @@ def test_hash_stage_bit_exact_packed_segments():
     s.close()
 
 
+def test_hash_stage_matches_golden_fixture():
+    say("hash golden fixture")
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hash_folded16.npz"))
+    c, p = _folded(16, seed=0)
+    c = S.grid_cloth(16, 0.05, center=(0.0, 1.0, 0.0), jitter=1e-4, seed=0, stream=7)
+    assert np.array_equal(c.x, g["x0"])
+    s = _hash_solver([(c, g["p"])], [int(g["M"])])
+    cs, ce, sh, si, nb = s.read_hash(0)
+    np.testing.assert_array_equal(cs, g["cell_start"])
+    valid = cs != SENT
+    np.testing.assert_array_equal(ce[valid], g["cell_end"][valid])
+    np.testing.assert_array_equal(si, g["sorted_index"])
+    assert oracle.neighbor_lists(nb) == oracle.neighbor_lists(g["nbr"])
+    s.close()
+
+
 def test_hash_caps_and_duplicates():
     say("hash caps/duplicates")
     n = 80
@@ -142,6 +159,9 @@ def test_config1_sixty_frames_vs_fp64(stencil):
     say(f"config1 60 frames stencil={stencil}")
     sc = S.config1(frames=60)
     x, v, nrm = _run_gpu(sc, 60, stencil=stencil, expect_stencil=int(stencil))
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config1_oracle.npz"))
+    assert np.max(np.abs(x.astype(np.float64) - g["sixty_frames"])) <= 1e-3 * sc.extent()
     ref = oracle.simulate(sc, frames=60)
     d = np.abs(x.astype(np.float64) - ref.x.reshape(-1, 3))
     err = float(np.max(d))
@@ -165,13 +185,12 @@ def test_config1_frames_reference_topologies(bend_mode, stencil):
 
 
 def _tiny(x, **kw):
-    from test_oracle_pins import tiny_scene
+    from tests.test_oracle_pins import tiny_scene
     return tiny_scene(x, **kw)
 
 
 def test_closed_forms_through_pipeline():
     say("closed forms")
-    sys.path.insert(0, __file__.rsplit("/", 1)[0])
     # free fall with damping
     sc = _tiny([[0, 1, 0]], num_substeps=5, damping=0.25, gravity=(0.0, -9.8, 0.0))
     x, v, _ = _run_gpu(sc, 3)
