@@ -152,6 +152,55 @@ def cpu_baseline(scenes, budget_s):
     return done_ps / spent, spent, frames, cloths
 
 
+def serve_run(rate, seconds, seed, max_live_particles, use_graph, ahead=4):
+    """Poisson arrivals at `rate` req/s for `seconds` of wall time against the serving loop.
+
+    Arrived requests wait in a backlog (descriptors only); the driver materialises and submits
+    them while the server's own queue holds fewer than `ahead` jobs, so host memory stays
+    bounded when arrivals exceed capacity. Step latency is the server's own wall time per step.
+    """
+    from velvet_amd.server import Server
+    rng = np.random.default_rng(seed)
+    lat, work, live = [], 0, []
+    backlog = []
+    arrived = submitted = admitted = completed = 0
+    k = 0
+    next_t = rng.exponential(1.0 / rate)
+    with Server(max_live_particles=max_live_particles, use_graph=use_graph) as sv:
+        queued = 0
+        t0 = time.perf_counter()
+        while True:
+            now = time.perf_counter() - t0
+            if now >= seconds:
+                break
+            while next_t <= now:
+                backlog.append(k)
+                k += 1
+                arrived += 1
+                next_t += rng.exponential(1.0 / rate)
+            while backlog and queued < ahead:
+                sc, horizon = S.serving_job(seed, backlog.pop(0))
+                sv.submit(sc, horizon)
+                submitted += 1
+                queued += 1
+            st = sv.step()
+            queued = st.queued_jobs
+            admitted += st.admitted
+            completed += st.completed
+            if st.live_jobs or st.completed:
+                lat.append(st.step_ms)
+                work += st.particle_substeps
+                live.append(st.live_particles)
+        wall = time.perf_counter() - t0
+    return {"rate_rps": rate, "seconds": round(wall, 2), "steps": len(lat),
+            "p50_ms": float(np.percentile(lat, 50)) if lat else None,
+            "p99_ms": float(np.percentile(lat, 99)) if lat else None,
+            "particle_substeps_per_s": work / wall, "arrived": arrived, "admitted": admitted,
+            "completed": completed, "backlog": len(backlog) + queued,
+            "mean_live_particles": float(np.mean(live)) if live else 0.0,
+            "max_live_particles_cap": max_live_particles}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +213,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-rates", default="50", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-cap", type=int, default=3_000_000, help="live-particle cap (latency budget)")
+    ap.add_argument("--serve-graph", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -215,6 +269,14 @@ def main():
                  "batched_over_plain": value / pval}
         log(f"plain: {pval:.4e} particle-substeps/s -> batched/plain {value / pval:.2f}x")
 
+    serving = None
+    if not args.no_serve:
+        serving = []
+        for r in [float(x) for x in args.serve_rates.split(",") if x]:
+            log(f"serving: {r:g} req/s for {args.serve_seconds:g} s, cap {args.serve_cap} live particles")
+            serving.append(serve_run(r, args.serve_seconds, 0, args.serve_cap, bool(args.serve_graph)))
+            log(f"  {serving[-1]}")
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_seconds:.0f} s budget)")
@@ -236,7 +298,7 @@ def main():
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "frames": len(lat)},
             "kernels_ms": {k: v for k, v in kt.items()},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "serving": serving, "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

@@ -89,8 +89,12 @@ int vtx_solver_destroy(vtx_solver* s);
 int vtx_set_params(vtx_solver* s, int32_t seg, const vtx_params* params);
 int vtx_get_params(vtx_solver* s, int32_t seg, vtx_params* out);
 
-/* Batching extension: add an independent segment; returns its index. */
+/* Batching extension: add an independent segment; returns its index (indices are never
+ * reused). A removed segment frees its device ranges; an inactive segment keeps its state
+ * resident but is skipped by vtx_simulate. Other segments are not moved or re-uploaded. */
 int vtx_add_segment(vtx_solver* s, const vtx_params* params, int32_t* seg_out);
+int vtx_remove_segment(vtx_solver* s, int32_t seg);
+int vtx_set_segment_active(vtx_solver* s, int32_t seg, int32_t active);
 int vtx_num_segments(vtx_solver* s, int32_t* out);
 
 /* Replaces int AddCloth(shared_ptr<Mesh>, glm::mat4, float) (Velvet/VtClothSolverGPU.hpp:114-156)
@@ -179,6 +183,51 @@ int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* ce
  * since timing was switched on. */
 int vtx_set_timing(vtx_solver* s, int32_t on);
 int vtx_read_timing(vtx_solver* s, char* buf, size_t len);
+
+/* ---------------------------------------------------------------------------------------
+ * Serving extension (no counterpart in the reference): continuous batching of simulation
+ * jobs. A job is a pending segment of the server's solver with a frame horizon; the caller
+ * registers its cloths / constraints / colliders on that segment through the calls above
+ * (vtx_server_solver gives the solver handle), then submits it. Each vtx_server_step admits
+ * queued jobs first-come-first-served while the device-memory budget, the live-particle cap
+ * and the job cap allow, advances every live job one frame in the same launches, and retires
+ * finished jobs (their final positions stay readable until vtx_server_release_job).
+ * ------------------------------------------------------------------------------------- */
+#define VTX_JOB_PENDING 0    /* created, not submitted */
+#define VTX_JOB_QUEUED 1
+#define VTX_JOB_RUNNING 2
+#define VTX_JOB_DONE 3
+#define VTX_JOB_FAILED 4     /* evicted: non-finite state detected */
+#define VTX_JOB_REJECTED 5   /* larger than the whole budget */
+
+typedef struct vtx_server_config {
+    uint64_t hbm_budget_bytes;     /* device bytes jobs may occupy; 0 = free HBM at creation */
+    uint64_t hbm_reserve_bytes;    /* kept free when the budget comes from free HBM */
+    uint64_t max_live_particles;   /* latency budget; 0 = none */
+    uint32_t max_live_jobs;        /* 0 = none */
+    int32_t use_graph;             /* replay frames as hipGraphs (re-recorded on membership change) */
+    int32_t check_every;           /* non-finite check period in steps; 0 = never */
+    int32_t pad;
+} vtx_server_config;
+
+typedef struct vtx_server_stats {
+    uint32_t live_jobs, queued_jobs, admitted, completed, failed, pad;
+    uint64_t live_particles;
+    uint64_t particle_substeps;    /* work done by this step */
+    double step_ms;                /* wall time of this step (admission + frame + retirement) */
+} vtx_server_stats;
+
+typedef struct vtx_server vtx_server;
+
+int vtx_server_create(const vtx_server_config* cfg, vtx_server** out);
+int vtx_server_destroy(vtx_server* sv);
+int vtx_server_solver(vtx_server* sv, vtx_solver** out);
+int vtx_server_new_job(vtx_server* sv, const vtx_params* params, int32_t frames, int32_t* job, int32_t* seg);
+int vtx_server_submit(vtx_server* sv, int32_t job);
+int vtx_server_step(vtx_server* sv, float frame_dt, vtx_server_stats* out);
+int vtx_server_job_status(vtx_server* sv, int32_t job, int32_t* status, int32_t* frames_done);
+int vtx_server_read_result(vtx_server* sv, int32_t job, float* xyz, uint32_t n);
+int vtx_server_release_job(vtx_server* sv, int32_t job);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,18 @@ class VtxCollider(C.Structure):
                 ("last", f32 * 16), ("delta_time", f32)]
 
 
+class VtxServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("hbm_reserve_bytes", C.c_uint64),
+                ("max_live_particles", C.c_uint64), ("max_live_jobs", u32), ("use_graph", i32),
+                ("check_every", i32), ("pad", i32)]
+
+
+class VtxServerStats(C.Structure):
+    _fields_ = [("live_jobs", u32), ("queued_jobs", u32), ("admitted", u32), ("completed", u32),
+                ("failed", u32), ("pad", u32), ("live_particles", C.c_uint64),
+                ("particle_substeps", C.c_uint64), ("step_ms", C.c_double)]
+
+
 _SIGS = {
     "vtx_params_default": (None, [P(VtxParams)]),
     "vtx_last_error": (C.c_char_p, []),
@@ -41,6 +53,8 @@ _SIGS = {
     "vtx_set_params": (i32, [C.c_void_p, i32, P(VtxParams)]),
     "vtx_get_params": (i32, [C.c_void_p, i32, P(VtxParams)]),
     "vtx_add_segment": (i32, [C.c_void_p, P(VtxParams), P(i32)]),
+    "vtx_remove_segment": (i32, [C.c_void_p, i32]),
+    "vtx_set_segment_active": (i32, [C.c_void_p, i32, i32]),
     "vtx_num_segments": (i32, [C.c_void_p, P(i32)]),
     "vtx_add_cloth": (i32, [C.c_void_p, i32, P(f32), u32, P(u32), u32, P(f32), f32, P(i32)]),
     "vtx_add_stretch": (i32, [C.c_void_p, i32, i32, i32, f32]),
@@ -72,6 +86,15 @@ _SIGS = {
     "vtx_read_hash": (i32, [C.c_void_p, i32, P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "vtx_set_timing": (i32, [C.c_void_p, i32]),
     "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "vtx_server_create": (i32, [P(VtxServerConfig), P(C.c_void_p)]),
+    "vtx_server_destroy": (i32, [C.c_void_p]),
+    "vtx_server_solver": (i32, [C.c_void_p, P(C.c_void_p)]),
+    "vtx_server_new_job": (i32, [C.c_void_p, P(VtxParams), i32, P(i32), P(i32)]),
+    "vtx_server_submit": (i32, [C.c_void_p, i32]),
+    "vtx_server_step": (i32, [C.c_void_p, f32, P(VtxServerStats)]),
+    "vtx_server_job_status": (i32, [C.c_void_p, i32, P(i32), P(i32)]),
+    "vtx_server_read_result": (i32, [C.c_void_p, i32, P(f32), u32]),
+    "vtx_server_release_job": (i32, [C.c_void_p, i32]),
 }
 
 _LIB = None

@@ -6,10 +6,17 @@
 
 #include "../../include/vtx.h"
 #include "engine.hpp"
+#include "server.hpp"
 
 struct vtx_solver {
     vtx::Engine engine;
     explicit vtx_solver(const vtx_params& p) : engine(p) {}
+};
+
+struct vtx_server {
+    vtx_solver solver;
+    vtx::Server server;
+    vtx_server(const vtx_params& p, const vtx_server_config& c) : solver(p), server(solver.engine, c) {}
 };
 
 namespace {
@@ -105,6 +112,16 @@ int vtx_add_segment(vtx_solver* s, const vtx_params* params, int32_t* seg_out) {
     int32_t k = s->engine.add_segment(*params);
     if (seg_out) *seg_out = k;
     API_END
+}
+
+int vtx_remove_segment(vtx_solver* s, int32_t seg) {
+    NEED(s);
+    API_BEGIN s->engine.remove_segment(seg); API_END
+}
+
+int vtx_set_segment_active(vtx_solver* s, int32_t seg, int32_t active) {
+    NEED(s);
+    API_BEGIN s->engine.set_active(seg, active != 0); API_END
 }
 
 int vtx_num_segments(vtx_solver* s, int32_t* out) {
@@ -277,6 +294,63 @@ int vtx_read_timing(vtx_solver* s, char* buf, size_t len) {
         buf[len - 1] = 0;
     }
     API_END
+}
+
+int vtx_server_create(const vtx_server_config* cfg, vtx_server** out) {
+    NEED(cfg); NEED(out);
+    API_BEGIN
+    vtx_params p;
+    vtx_params_default(&p);
+    *out = new vtx_server(p, *cfg);
+    API_END
+}
+
+int vtx_server_destroy(vtx_server* sv) {
+    API_BEGIN delete sv; API_END
+}
+
+int vtx_server_solver(vtx_server* sv, vtx_solver** out) {
+    NEED(sv); NEED(out);
+    API_BEGIN *out = &sv->solver; API_END
+}
+
+int vtx_server_new_job(vtx_server* sv, const vtx_params* params, int32_t frames, int32_t* job, int32_t* seg) {
+    NEED(sv); NEED(params); NEED(job);
+    API_BEGIN *job = sv->server.new_job(*params, frames, seg); API_END
+}
+
+int vtx_server_submit(vtx_server* sv, int32_t job) {
+    NEED(sv);
+    API_BEGIN sv->server.submit(job); API_END
+}
+
+int vtx_server_step(vtx_server* sv, float frame_dt, vtx_server_stats* out) {
+    NEED(sv);
+    API_BEGIN sv->server.step(frame_dt, out); API_END
+}
+
+int vtx_server_job_status(vtx_server* sv, int32_t job, int32_t* status, int32_t* frames_done) {
+    NEED(sv);
+    API_BEGIN
+    const vtx::Job& j = sv->server.job(job);
+    if (status) *status = j.status;
+    if (frames_done) *frames_done = j.done;
+    API_END
+}
+
+int vtx_server_read_result(vtx_server* sv, int32_t job, float* xyz, uint32_t n) {
+    NEED(sv); NEED(xyz);
+    API_BEGIN
+    const vtx::Job& j = sv->server.job(job);
+    if (j.status != VTX_JOB_DONE) throw vtx::Error(VTX_ESTATE, "job has no result");
+    if (n != j.n) throw vtx::Error(VTX_EINVAL, "particle count mismatch");
+    std::memcpy(xyz, j.result.data(), sizeof(float) * 3 * (size_t)n);
+    API_END
+}
+
+int vtx_server_release_job(vtx_server* sv, int32_t job) {
+    NEED(sv);
+    API_BEGIN sv->server.release(job); API_END
 }
 
 }  // extern "C"

@@ -1,6 +1,8 @@
-// Host engine: registration mirrors VtClothSolverGPU (Velvet/VtClothSolverGPU.hpp:114-205),
-// commit() packs every segment into the device layout of vtx_device.hpp, simulate() records
-// one frame (Velvet/VtClothSolverGPU.hpp:56-111) and replays it as a hipGraph.
+// Host engine: registration mirrors VtClothSolverGPU (Velvet/VtClothSolverGPU.hpp:114-205).
+// Segments are prepared on the host with segment-local indices, uploaded into ranges of
+// device pools (so segments can join and leave a running solver without touching the others),
+// and every frame (Velvet/VtClothSolverGPU.hpp:56-111) is recorded once and replayed as a
+// hipGraph until the set of active segments changes.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -15,205 +17,49 @@ namespace vtx {
 
 static const float kFixedDt = 1.0f / 60.0f;   // Velvet/Timer.hpp:235
 
-Engine::Engine(const vtx_params& p) {
-    VTX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    own_stream_ = true;
-    add_segment(p);
-}
-
-Engine::~Engine() {
-    if (stream_) hipStreamSynchronize(stream_);
-    if (gexec_) hipGraphExecDestroy(gexec_);
-    for (auto& st : stages_) {
-        hipEventDestroy(st.a);
-        hipEventDestroy(st.b);
+// ---------------------------------------------------------------------------- ranges
+bool RangeAlloc::take(size_t n, size_t* off) {
+    if (n == 0) {
+        *off = 0;
+        return true;
     }
-    free_device();
-    if (own_stream_ && stream_) hipStreamDestroy(stream_);
-}
-
-void Engine::check_seg(int32_t s) const {
-    VTX_REQUIRE(s >= 0 && s < (int32_t)segs_.size(), VTX_EINVAL, "segment index out of range");
-}
-
-Segment& Engine::seg(int32_t i) {
-    check_seg(i);
-    return segs_[(size_t)i];
-}
-
-static void validate_params(const vtx_params& p) {
-    VTX_REQUIRE(p.num_substeps >= 1, VTX_EINVAL, "num_substeps must be >= 1");
-    VTX_REQUIRE(p.num_iterations >= 0, VTX_EINVAL, "num_iterations must be >= 0");
-    VTX_REQUIRE(p.max_num_neighbors >= 1 && p.max_num_neighbors <= VTX_MAX_NEIGHBORS, VTX_EINVAL,
-                "max_num_neighbors must be in [1, 64]");
-    VTX_REQUIRE(p.interleaved_hash >= 1, VTX_EINVAL, "interleaved_hash must be >= 1");
-    VTX_REQUIRE(p.hash_table_size >= 0, VTX_EINVAL, "hash_table_size must be >= 0");
-}
-
-int32_t Engine::add_segment(const vtx_params& p) {
-    validate_params(p);
-    pull_state();
-    Segment s;
-    s.prm = p;
-    segs_.push_back(std::move(s));
-    mark_dirty();
-    return (int32_t)segs_.size() - 1;
-}
-
-void Engine::set_params(int32_t s, const vtx_params& p) {
-    validate_params(p);
-    pull_state();
-    seg(s).prm = p;
-    mark_dirty();
-}
-
-// Device -> host copy of the particle state before a structural change.
-void Engine::pull_state() {
-    if (!dev_live_ || dirty_) return;
-    VTX_HIP(hipStreamSynchronize(stream_));
-    for (size_t k = 0; k < segs_.size(); k++) {
-        Segment& sg = segs_[k];
-        const uint32_t n = sg.n();
-        if (!n) continue;
-        std::vector<float4> tmp(n);
-        const uint32_t base = hseg_[k].pbase;
-        VTX_HIP(hipMemcpy(tmp.data(), ds_.X + base, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; i++) {
-            sg.x[3 * i] = tmp[i].x;
-            sg.x[3 * i + 1] = tmp[i].y;
-            sg.x[3 * i + 2] = tmp[i].z;
-            sg.w[i] = tmp[i].w;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= n) {
+            *off = it->first;
+            const size_t rest = it->second - n, o = it->first + n;
+            free_.erase(it);
+            if (rest) free_[o] = rest;
+            used_ += n;
+            return true;
         }
-        VTX_HIP(hipMemcpy(tmp.data(), ds_.V + base, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; i++) {
-            sg.v[3 * i] = tmp[i].x;
-            sg.v[3 * i + 1] = tmp[i].y;
-            sg.v[3 * i + 2] = tmp[i].z;
+    }
+    return false;
+}
+
+void RangeAlloc::give(size_t off, size_t n) {
+    if (n == 0) return;
+    used_ -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
         }
     }
 }
 
-void Engine::mark_dirty() {
-    dirty_ = true;
-    if (gexec_) {
-        hipGraphExecDestroy(gexec_);
-        gexec_ = nullptr;
-    }
-}
-
-// AddCloth (Velvet/VtClothSolverGPU.hpp:114-156).
-int32_t Engine::add_cloth(int32_t s, const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
-                          const float* model, float diameter) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(n == 0 || xyz, VTX_EINVAL, "positions are NULL");
-    VTX_REQUIRE(ntri == 0 || tris, VTX_EINVAL, "triangles are NULL");
-    for (uint32_t t = 0; t < 3 * ntri; t++) VTX_REQUIRE(tris[t] < n, VTX_EINVAL, "triangle index out of range");
-    pull_state();
-    const uint32_t prev = sg.n();
-    sg.prm.particle_diameter = diameter;
-    sg.prm.max_speed = 2.0f * diameter / kFixedDt * (float)sg.prm.num_substeps;
-    for (uint32_t t = 0; t < 3 * ntri; t++) sg.tris.push_back(tris[t] + prev);
-    for (uint32_t i = 0; i < n; i++) {
-        float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
-        if (model) {   // InitializePositions_Kernel: x = M * vec4(x, 1) (Velvet/VtClothSolverGPU.cu:30-34)
-            const float* m = model;
-            float ox = m[0] * px + m[4] * py + m[8] * pz + m[12];
-            float oy = m[1] * px + m[5] * py + m[9] * pz + m[13];
-            float oz = m[2] * px + m[6] * py + m[10] * pz + m[14];
-            px = ox;
-            py = oy;
-            pz = oz;
-        }
-        sg.x.push_back(px);
-        sg.x.push_back(py);
-        sg.x.push_back(pz);
-        sg.v.insert(sg.v.end(), {0.f, 0.f, 0.f});
-        sg.w.push_back(1.0f);
-    }
-    sg.cloth_starts.push_back(prev);
-    sg.x0 = sg.x;   // SetInitialPositions snapshot of every particle (Velvet/SpatialHashGPU.hpp:32-39)
-    mark_dirty();
-    return (int32_t)prev;
-}
-
-void Engine::add_stretch(int32_t s, const int32_t* ij, const float* rest, uint32_t count) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(count == 0 || (ij && rest), VTX_EINVAL, "stretch arrays are NULL");
-    const int32_t n = (int32_t)sg.n();
-    for (uint32_t c = 0; c < 2 * count; c++)
-        VTX_REQUIRE(ij[c] >= 0 && ij[c] < n, VTX_EINVAL, "stretch index out of range");
-    pull_state();
-    sg.st.insert(sg.st.end(), ij, ij + 2 * count);
-    sg.st_rest.insert(sg.st_rest.end(), rest, rest + count);
-    mark_dirty();
-}
-
-int32_t Engine::add_attach_slot(int32_t s, const float* pos) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
-    pull_state();
-    sg.slots.insert(sg.slots.end(), pos, pos + 3);
-    mark_dirty();
-    return (int32_t)(sg.slots.size() / 3 - 1);
-}
-
-void Engine::set_attach_slot_position(int32_t s, int32_t slot, const float* pos) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
-    VTX_REQUIRE(slot >= 0 && (size_t)slot < sg.slots.size() / 3, VTX_EINVAL, "slot index out of range");
-    std::copy(pos, pos + 3, sg.slots.begin() + 3 * slot);
-    if (!dirty_) {
-        float4 v = make_float4(pos[0], pos[1], pos[2], 0.f);
-        VTX_HIP(hipMemcpyAsync((float4*)ds_.slots + hseg_[(size_t)s].sbase + slot, &v, sizeof(float4),
-                               hipMemcpyHostToDevice, stream_));
-        VTX_HIP(hipStreamSynchronize(stream_));
-    }
-}
-
-// AddAttach (Velvet/VtClothSolverGPU.hpp:170-176): distance 0 pins the particle.
-void Engine::add_attach(int32_t s, const int32_t* pid, const int32_t* slot, const float* dist, uint32_t count) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(count == 0 || (pid && slot && dist), VTX_EINVAL, "attach arrays are NULL");
-    const int32_t n = (int32_t)sg.n();
-    const int32_t L = (int32_t)(sg.slots.size() / 3);
-    for (uint32_t c = 0; c < count; c++) {
-        VTX_REQUIRE(pid[c] >= 0 && pid[c] < n, VTX_EINVAL, "attach particle out of range");
-        VTX_REQUIRE(slot[c] >= 0 && slot[c] < L, VTX_EINVAL, "attach slot out of range");
-    }
-    pull_state();
-    for (uint32_t c = 0; c < count; c++) {
-        if (dist[c] == 0.0f) sg.w[(size_t)pid[c]] = 0.0f;
-        sg.at_pid.push_back(pid[c]);
-        sg.at_slot.push_back(slot[c]);
-        sg.at_dist.push_back(dist[c]);
-    }
-    mark_dirty();
-}
-
-void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(count == 0 || (idx && angle), VTX_EINVAL, "bend arrays are NULL");
-    VTX_REQUIRE(sg.bd.size() / 4 + count < (1u << 28), VTX_ECAPACITY, "too many bend constraints");
-    for (uint32_t c = 0; c < 4 * count; c++) VTX_REQUIRE(idx[c] < sg.n(), VTX_EINVAL, "bend index out of range");
-    pull_state();
-    sg.bd.insert(sg.bd.end(), idx, idx + 4 * count);
-    sg.bd_rest.insert(sg.bd_rest.end(), angle, angle + count);
-    mark_dirty();
-}
-
-void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
-    Segment& sg = seg(s);
-    VTX_REQUIRE(n == 0 || cols, VTX_EINVAL, "colliders are NULL");
-    for (uint32_t c = 0; c < n; c++)
-        VTX_REQUIRE(cols[c].type >= 0 && cols[c].type <= 2, VTX_EINVAL, "unknown collider type");
-    const bool same_count = sg.cols.size() == n;
-    sg.cols.assign(cols, cols + n);
-    if (same_count && !dirty_) {
-        upload_colliders();   // the graph reads the collider table; no re-record needed
-    } else {
-        pull_state();
-        mark_dirty();
-    }
+void RangeAlloc::grow(size_t new_cap) {
+    if (new_cap <= cap) return;
+    const size_t add = new_cap - cap;
+    used_ += add;            // give() subtracts it again: the new space starts free
+    give(cap, add);
+    cap = new_cap;
 }
 
 // ---------------------------------------------------------------------------- colliders
@@ -270,49 +116,6 @@ static ColliderDev pack_collider(const vtx_collider& c) {
             d.rel[r * 4 + col] = (float)rel[col * 4 + r];
         }
     return d;
-}
-
-void Engine::upload_colliders() {
-    std::vector<ColliderDev> h;
-    for (auto& sg : segs_)
-        for (auto& c : sg.cols) h.push_back(pack_collider(c));
-    if (h.empty()) return;
-    VTX_HIP(hipMemcpyAsync((void*)ds_.cols, h.data(), sizeof(ColliderDev) * h.size(), hipMemcpyHostToDevice,
-                           stream_));
-    VTX_HIP(hipStreamSynchronize(stream_));
-}
-
-// ---------------------------------------------------------------------------- device layout
-template <class T>
-T* Engine::dalloc(size_t count) {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(sizeof(T) * count, 256);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        hipGetLastError();
-        throw Error(VTX_ENOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    }
-    allocs_.push_back(p);
-    return (T*)p;
-}
-
-template <class T>
-T* Engine::dupload(const std::vector<T>& h) {
-    T* p = dalloc<T>(h.size());
-    if (!h.empty()) VTX_HIP(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-    return p;
-}
-
-void Engine::free_device() {
-    for (void* p : allocs_) hipFree(p);
-    allocs_.clear();
-    ds_ = DevState{};
-}
-
-static uint32_t ceil_log2(uint32_t v) {
-    uint32_t b = 0;
-    while ((1ull << b) < v) b++;
-    return b;
 }
 
 // Sliced-ELL packing of per-particle lists for one tile: entries [off + k * VTX_TILE + lane].
@@ -447,307 +250,781 @@ static bool detect_grid(const Segment& sg, GridLayout& G) {
     return true;
 }
 
+// ---------------------------------------------------------------------------- lifecycle
+Engine::Engine(const vtx_params& p) {
+    VTX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+    add_segment(p);
+}
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    for (auto& st : stages_) {
+        (void)hipEventDestroy(st.a);
+        (void)hipEventDestroy(st.b);
+    }
+    free_tables();
+    free_pools();
+    if (hist_) (void)hipFree(hist_);
+    if (part_) (void)hipFree(part_);
+    if (flags_) (void)hipFree(flags_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::check_seg(int32_t s) const {
+    VTX_REQUIRE(s >= 0 && s < (int32_t)segs_.size() && segs_[(size_t)s], VTX_EINVAL, "no such segment");
+}
+
+Segment& Engine::seg(int32_t i) {
+    check_seg(i);
+    return *segs_[(size_t)i];
+}
+
+int32_t Engine::num_segments() const {
+    int32_t k = 0;
+    for (auto& p : segs_) k += p ? 1 : 0;
+    return k;
+}
+
+static void validate_params(const vtx_params& p) {
+    VTX_REQUIRE(p.num_substeps >= 1, VTX_EINVAL, "num_substeps must be >= 1");
+    VTX_REQUIRE(p.num_iterations >= 0, VTX_EINVAL, "num_iterations must be >= 0");
+    VTX_REQUIRE(p.max_num_neighbors >= 1 && p.max_num_neighbors <= VTX_MAX_NEIGHBORS, VTX_EINVAL,
+                "max_num_neighbors must be in [1, 64]");
+    VTX_REQUIRE(p.interleaved_hash >= 1, VTX_EINVAL, "interleaved_hash must be >= 1");
+    VTX_REQUIRE(p.hash_table_size >= 0, VTX_EINVAL, "hash_table_size must be >= 0");
+}
+
+int32_t Engine::add_segment(const vtx_params& p, bool active) {
+    validate_params(p);
+    auto s = std::make_unique<Segment>();
+    s->prm = p;
+    s->active = active;
+    segs_.push_back(std::move(s));
+    if (active) tables_dirty_ = true;
+    return (int32_t)segs_.size() - 1;
+}
+
+void Engine::remove_segment(int32_t s) {
+    Segment& sg = seg(s);
+    if (sg.uploaded) {
+        synchronize();
+        release(sg);
+    }
+    if (sg.active) tables_dirty_ = true;
+    segs_[(size_t)s].reset();
+}
+
+void Engine::set_active(int32_t s, bool active) {
+    Segment& sg = seg(s);
+    if (sg.active == active) return;
+    sg.active = active;
+    tables_dirty_ = true;
+}
+
+void Engine::invalidate_graph() {
+    if (gexec_) {
+        (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+    }
+}
+
+// Device -> host copy of one segment's particle state.
+void Engine::pull_state(Segment& sg) {
+    if (!sg.uploaded) return;
+    VTX_HIP(hipStreamSynchronize(stream_));
+    const uint32_t n = sg.n();
+    if (!n) return;
+    std::vector<float4> tmp(n);
+    VTX_HIP(hipMemcpy(tmp.data(), pl_.X + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
+        sg.x[3 * i] = tmp[i].x;
+        sg.x[3 * i + 1] = tmp[i].y;
+        sg.x[3 * i + 2] = tmp[i].z;
+        sg.w[i] = tmp[i].w;
+    }
+    VTX_HIP(hipMemcpy(tmp.data(), pl_.V + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
+        sg.v[3 * i] = tmp[i].x;
+        sg.v[3 * i + 1] = tmp[i].y;
+        sg.v[3 * i + 2] = tmp[i].z;
+    }
+}
+
+// A structural change of one segment: keep its simulated state on the host, drop its device
+// ranges and prepared layout; other segments are untouched.
+void Engine::touch(int32_t s) {
+    Segment& sg = seg(s);
+    if (sg.uploaded) {
+        pull_state(sg);
+        release(sg);
+        if (sg.active) tables_dirty_ = true;
+    }
+    sg.prepared = false;
+}
+
+void Engine::set_params(int32_t s, const vtx_params& p) {
+    validate_params(p);
+    touch(s);
+    seg(s).prm = p;
+}
+
+// AddCloth (Velvet/VtClothSolverGPU.hpp:114-156).
+int32_t Engine::add_cloth(int32_t s, const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
+                          const float* model, float diameter) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(n == 0 || xyz, VTX_EINVAL, "positions are NULL");
+    VTX_REQUIRE(ntri == 0 || tris, VTX_EINVAL, "triangles are NULL");
+    for (uint32_t t = 0; t < 3 * ntri; t++) VTX_REQUIRE(tris[t] < n, VTX_EINVAL, "triangle index out of range");
+    touch(s);
+    const uint32_t prev = sg.n();
+    sg.prm.particle_diameter = diameter;
+    sg.prm.max_speed = 2.0f * diameter / kFixedDt * (float)sg.prm.num_substeps;
+    for (uint32_t t = 0; t < 3 * ntri; t++) sg.tris.push_back(tris[t] + prev);
+    for (uint32_t i = 0; i < n; i++) {
+        float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+        if (model) {   // InitializePositions_Kernel: x = M * vec4(x, 1) (Velvet/VtClothSolverGPU.cu:30-34)
+            const float* m = model;
+            const float ox = m[0] * px + m[4] * py + m[8] * pz + m[12];
+            const float oy = m[1] * px + m[5] * py + m[9] * pz + m[13];
+            const float oz = m[2] * px + m[6] * py + m[10] * pz + m[14];
+            px = ox;
+            py = oy;
+            pz = oz;
+        }
+        sg.x.push_back(px);
+        sg.x.push_back(py);
+        sg.x.push_back(pz);
+        sg.v.insert(sg.v.end(), {0.f, 0.f, 0.f});
+        sg.w.push_back(1.0f);
+    }
+    sg.cloth_starts.push_back(prev);
+    sg.x0 = sg.x;   // SetInitialPositions snapshot of every particle (Velvet/SpatialHashGPU.hpp:32-39)
+    return (int32_t)prev;
+}
+
+void Engine::add_stretch(int32_t s, const int32_t* ij, const float* rest, uint32_t count) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(count == 0 || (ij && rest), VTX_EINVAL, "stretch arrays are NULL");
+    const int32_t n = (int32_t)sg.n();
+    for (uint32_t c = 0; c < 2 * count; c++)
+        VTX_REQUIRE(ij[c] >= 0 && ij[c] < n, VTX_EINVAL, "stretch index out of range");
+    touch(s);
+    sg.st.insert(sg.st.end(), ij, ij + 2 * count);
+    sg.st_rest.insert(sg.st_rest.end(), rest, rest + count);
+}
+
+int32_t Engine::add_attach_slot(int32_t s, const float* pos) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
+    touch(s);
+    sg.slots.insert(sg.slots.end(), pos, pos + 3);
+    return (int32_t)(sg.slots.size() / 3 - 1);
+}
+
+void Engine::set_attach_slot_position(int32_t s, int32_t slot, const float* pos) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
+    VTX_REQUIRE(slot >= 0 && (size_t)slot < sg.slots.size() / 3, VTX_EINVAL, "slot index out of range");
+    std::copy(pos, pos + 3, sg.slots.begin() + 3 * slot);
+    const float4 v = make_float4(pos[0], pos[1], pos[2], 0.f);
+    if (sg.prepared) sg.prep.slots[(size_t)slot] = v;
+    if (sg.uploaded) {   // the graph reads the slot table: no re-record
+        VTX_HIP(hipMemcpyAsync(pl_.slots + sg.rg.slot + slot, &v, sizeof(float4), hipMemcpyHostToDevice, stream_));
+        VTX_HIP(hipStreamSynchronize(stream_));
+    }
+}
+
+// AddAttach (Velvet/VtClothSolverGPU.hpp:170-176): distance 0 pins the particle.
+void Engine::add_attach(int32_t s, const int32_t* pid, const int32_t* slot, const float* dist, uint32_t count) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(count == 0 || (pid && slot && dist), VTX_EINVAL, "attach arrays are NULL");
+    const int32_t n = (int32_t)sg.n();
+    const int32_t L = (int32_t)(sg.slots.size() / 3);
+    for (uint32_t c = 0; c < count; c++) {
+        VTX_REQUIRE(pid[c] >= 0 && pid[c] < n, VTX_EINVAL, "attach particle out of range");
+        VTX_REQUIRE(slot[c] >= 0 && slot[c] < L, VTX_EINVAL, "attach slot out of range");
+    }
+    touch(s);
+    for (uint32_t c = 0; c < count; c++) {
+        if (dist[c] == 0.0f) sg.w[(size_t)pid[c]] = 0.0f;
+        sg.at_pid.push_back(pid[c]);
+        sg.at_slot.push_back(slot[c]);
+        sg.at_dist.push_back(dist[c]);
+    }
+}
+
+void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(count == 0 || (idx && angle), VTX_EINVAL, "bend arrays are NULL");
+    VTX_REQUIRE(sg.bd.size() / 4 + count < (1u << 28), VTX_ECAPACITY, "too many bend constraints");
+    for (uint32_t c = 0; c < 4 * count; c++) VTX_REQUIRE(idx[c] < sg.n(), VTX_EINVAL, "bend index out of range");
+    touch(s);
+    sg.bd.insert(sg.bd.end(), idx, idx + 4 * count);
+    sg.bd_rest.insert(sg.bd_rest.end(), angle, angle + count);
+}
+
+void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(n == 0 || cols, VTX_EINVAL, "colliders are NULL");
+    for (uint32_t c = 0; c < n; c++)
+        VTX_REQUIRE(cols[c].type >= 0 && cols[c].type <= 2, VTX_EINVAL, "unknown collider type");
+    if (sg.cols.size() == n && sg.prepared) {   // same count: rewrite in place, no re-record
+        sg.cols.assign(cols, cols + n);
+        for (uint32_t c = 0; c < n; c++) sg.prep.cols[c] = pack_collider(cols[c]);
+        if (sg.uploaded && n) {
+            VTX_HIP(hipMemcpyAsync(pl_.cols + sg.rg.col, sg.prep.cols.data(), sizeof(ColliderDev) * n,
+                                   hipMemcpyHostToDevice, stream_));
+            VTX_HIP(hipStreamSynchronize(stream_));
+        }
+        return;
+    }
+    touch(s);
+    sg.cols.assign(cols, cols + n);
+}
+
+// ---------------------------------------------------------------------------- preparation
+// Host packing of one segment into its position-independent device layout.
+void Engine::prepare(int32_t si) {
+    Segment& sg = seg(si);
+    if (sg.prepared) return;
+    Prepared pr;
+    const uint32_t n = sg.n();
+    VTX_REQUIRE(!sg.selfcol() || n < (1u << 25), VTX_ECAPACITY,
+                "self collision supports at most 2^25 particles per segment");
+    pr.x.resize(n);
+    pr.v.resize(n);
+    pr.x0.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+        pr.x[i] = make_float4(sg.x[3 * i], sg.x[3 * i + 1], sg.x[3 * i + 2], sg.w[i]);
+        pr.v[i] = make_float4(sg.v[3 * i], sg.v[3 * i + 1], sg.v[3 * i + 2], 0.f);
+        pr.x0[i] = make_float4(sg.x0[3 * i], sg.x0[3 * i + 1], sg.x0[3 * i + 2], 0.f);
+    }
+    const uint32_t L = (uint32_t)(sg.slots.size() / 3);
+    for (uint32_t k = 0; k < L; k++)
+        pr.slots.push_back(make_float4(sg.slots[3 * k], sg.slots[3 * k + 1], sg.slots[3 * k + 2], 0.f));
+    for (auto& c : sg.cols) pr.cols.push_back(pack_collider(c));
+
+    GridLayout G;
+    pr.grid = use_stencil_ && detect_grid(sg, G);
+    if (pr.grid) {
+        pr.grows = G.rows;
+        pr.gcols = G.cols;
+        pr.grid_flags = VTX_SEG_GRID | (G.rest0 ? VTX_SEG_BEND_REST0 : 0u);
+        for (int type = 0; type < 3; type++) {   // all geometrically possible slots present, or none
+            bool full = true, none = true;
+            for (uint32_t a = 0; a < G.rows && (full || none); a++)
+                for (uint32_t b = 0; b < G.cols; b++) {
+                    const bool possible = a + 1 < G.rows && b + 1 < G.cols && (type != 1 || a >= 1) && (type != 2 || b >= 1);
+                    const bool present = (G.flags[(size_t)a * G.cols + b] >> type) & 1u;
+                    if (present != possible) full = false;
+                    if (present) none = false;
+                }
+            if (full) pr.grid_flags |= VTX_SEG_FULL_X << type;
+            if (none) pr.grid_flags |= VTX_SEG_NONE_X << type;
+        }
+        pr.rest4 = std::move(G.rest4);
+        pr.gflags = std::move(G.flags);
+        pr.brest4 = std::move(G.brest4);
+    }
+
+    // incidence lists (CSR over local particles, registration order)
+    std::vector<uint32_t> s_st(n + 1, 0), s_at(n + 1, 0), s_bd(n + 1, 0), s_tr(n + 1, 0);
+    const size_t S = sg.st_rest.size(), A = sg.at_pid.size(), B = sg.bd_rest.size(), T = sg.tris.size() / 3;
+    for (size_t c = 0; c < S; c++) {
+        s_st[(size_t)sg.st[2 * c] + 1]++;
+        s_st[(size_t)sg.st[2 * c + 1] + 1]++;
+    }
+    for (size_t c = 0; c < A; c++) s_at[(size_t)sg.at_pid[c] + 1]++;
+    for (size_t c = 0; c < B; c++)
+        for (int r = 0; r < 4; r++) s_bd[(size_t)sg.bd[4 * c + r] + 1]++;
+    for (size_t c = 0; c < T; c++)
+        for (int r = 0; r < 3; r++) s_tr[(size_t)sg.tris[3 * c + r] + 1]++;
+    for (uint32_t i = 0; i < n; i++) {
+        s_st[i + 1] += s_st[i];
+        s_at[i + 1] += s_at[i];
+        s_bd[i + 1] += s_bd[i];
+        s_tr[i + 1] += s_tr[i];
+    }
+    std::vector<uint2> l_st(s_st[n]), l_at(s_at[n]);
+    std::vector<uint32_t> l_bd(s_bd[n]), l_tr(s_tr[n]);
+    if (!pr.grid) {
+        std::vector<uint32_t> f(s_st.begin(), s_st.end() - 1);
+        for (size_t c = 0; c < S; c++) {
+            const uint32_t a = (uint32_t)sg.st[2 * c], b = (uint32_t)sg.st[2 * c + 1];
+            const uint32_t r = fbits(sg.st_rest[c]);
+            l_st[f[a]++] = make_uint2(b, r);
+            l_st[f[b]++] = make_uint2(a, r);
+        }
+        std::vector<uint32_t> fb(s_bd.begin(), s_bd.end() - 1);
+        for (size_t c = 0; c < B; c++)
+            for (uint32_t r = 0; r < 4; r++) l_bd[fb[sg.bd[4 * c + r]]++] = (uint32_t)c | (r << 30);
+        for (size_t c = 0; c < B; c++) {
+            pr.brec.push_back(make_uint4(sg.bd[4 * c], sg.bd[4 * c + 1], sg.bd[4 * c + 2], sg.bd[4 * c + 3]));
+            pr.brest.push_back(sg.bd_rest[c]);
+        }
+    }
+    {
+        std::vector<uint32_t> f(s_at.begin(), s_at.end() - 1);
+        for (size_t c = 0; c < A; c++)
+            l_at[f[(size_t)sg.at_pid[c]]++] = make_uint2((uint32_t)sg.at_slot[c], fbits(sg.at_dist[c]));
+    }
+    {
+        std::vector<uint32_t> f(s_tr.begin(), s_tr.end() - 1);
+        for (size_t c = 0; c < T; c++)
+            for (int r = 0; r < 3; r++) l_tr[f[sg.tris[3 * c + r]]++] = (uint32_t)c;
+        for (size_t c = 0; c < T; c++)
+            pr.trec.push_back(make_uint4(sg.tris[3 * c], sg.tris[3 * c + 1], sg.tris[3 * c + 2], 0));
+    }
+    for (uint32_t first = 0; first < n; first += VTX_TILE) {
+        TileDev t{};
+        t.begin = first;                   // local; made absolute in the tables
+        t.count = std::min<uint32_t>(VTX_TILE, n - first);
+        if (!pr.grid) {   // the stencil path keeps its own attach lists and needs no stretch/bend lists
+            pack_tile(pr.est, s_st, l_st, first, t.count, make_uint2(VTX_INVALID, 0u), &t.st_off, &t.st_k);
+            pack_tile(pr.eat, s_at, l_at, first, t.count, make_uint2(VTX_INVALID, 0u), &t.at_off, &t.at_k);
+            pack_tile(pr.ebd, s_bd, l_bd, first, t.count, VTX_INVALID, &t.bd_off, &t.bd_k);
+        }
+        pack_tile(pr.etr, s_tr, l_tr, first, t.count, VTX_INVALID, &t.tr_off, &t.tr_k);
+        pr.tiles.push_back(t);
+    }
+    if (pr.grid) {
+        for (uint32_t a0 = 0; a0 < pr.grows; a0 += VTX_GT)
+            for (uint32_t b0 = 0; b0 < pr.gcols; b0 += VTX_GT) {
+                GridTileDev gt{};
+                gt.a0 = a0;
+                gt.b0 = b0;
+                std::vector<int64_t> lanes(VTX_TILE, -1);
+                for (uint32_t l = 0; l < VTX_TILE; l++) {
+                    const uint32_t a = a0 + l / VTX_GT, b = b0 + l % VTX_GT;
+                    if (a < pr.grows && b < pr.gcols) lanes[l] = (int64_t)a * pr.gcols + b;
+                }
+                pack_lanes(pr.eat, s_at, l_at, lanes, make_uint2(VTX_INVALID, 0u), &gt.at_off, &gt.at_k);
+                pr.gtiles.push_back(gt);
+            }
+    }
+    sg.prep = std::move(pr);
+    sg.prepared = true;
+}
+
+static Ranges needs_of(const Segment& sg) {
+    const Prepared& pr = sg.prep;
+    Ranges r;
+    r.n_p = sg.n();
+    r.n_cell = sg.selfcol() ? sg.table_size() : 0;
+    r.n_nbr = sg.selfcol() ? (size_t)sg.prm.max_num_neighbors * sg.n() : 0;
+    r.n_st = pr.est.size();
+    r.n_at = pr.eat.size();
+    r.n_bd = pr.ebd.size();
+    r.n_tr = pr.etr.size();
+    r.n_brec = pr.brec.size();
+    r.n_trec = pr.trec.size();
+    r.n_slot = pr.slots.size();
+    r.n_col = pr.cols.size();
+    return r;
+}
+
+static size_t bytes_of(const Ranges& r) {
+    return r.n_p * (10 * sizeof(float4) + 5 * sizeof(uint32_t)) + r.n_cell * sizeof(uint2) + r.n_nbr * 4 +
+           (r.n_st + r.n_at) * sizeof(uint2) + (r.n_bd + r.n_tr) * 4 + r.n_brec * (sizeof(uint4) + 4) +
+           r.n_trec * sizeof(uint4) + r.n_slot * sizeof(float4) + r.n_col * sizeof(ColliderDev);
+}
+
+size_t Engine::device_bytes_estimate(int32_t s) {
+    prepare(s);
+    return bytes_of(needs_of(seg(s)));
+}
+
+size_t Engine::pool_bytes() const {
+    Ranges c;
+    c.n_p = ap_.cap;
+    c.n_cell = acell_.cap;
+    c.n_nbr = anbr_.cap;
+    c.n_st = ast_.cap;
+    c.n_at = aat_.cap;
+    c.n_bd = abd_.cap;
+    c.n_tr = atr_.cap;
+    c.n_brec = abrec_.cap;
+    c.n_trec = atrec_.cap;
+    c.n_slot = aslot_.cap;
+    c.n_col = acol_.cap;
+    return bytes_of(c);
+}
+
+// ---------------------------------------------------------------------------- pools
+template <class T>
+static void regrow(T*& p, size_t old_cap, size_t new_cap) {
+    T* q = nullptr;
+    const size_t bytes = std::max<size_t>(sizeof(T) * new_cap, 256);
+    hipError_t e = hipMalloc((void**)&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(VTX_ENOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    if (p && old_cap) VTX_HIP(hipMemcpy(q, p, sizeof(T) * old_cap, hipMemcpyDeviceToDevice));
+    if (p) VTX_HIP(hipFree(p));
+    p = q;
+}
+
+static size_t next_cap(const RangeAlloc& a, size_t need_more) {
+    const size_t want = a.used() + need_more;
+    size_t c = std::max<size_t>(a.cap, 1);
+    while (c < want) c = c + c / 2 + 1;
+    return c;
+}
+
+// Make every pool large enough for `need` more entries (existing contents are kept in place).
+void Engine::grow_pools(const Ranges& need) {
+    auto grow = [&](RangeAlloc& a, size_t more, auto&& move) {
+        // first-fit may fail on fragmentation even when used + more <= cap: grow by `more` then
+        size_t off;
+        if (a.take(more, &off)) {
+            a.give(off, more);
+            return;
+        }
+        const size_t nc = std::max(next_cap(a, more), a.cap + more);
+        move(a.cap, nc);
+        a.grow(nc);
+        invalidate_graph();
+        tables_dirty_ = true;
+    };
+    grow(ap_, need.n_p, [&](size_t o, size_t c) {
+        regrow(pl_.X, o, c); regrow(pl_.V, o, c); regrow(pl_.P0, o, c); regrow(pl_.P1, o, c);
+        regrow(pl_.X0, o, c); regrow(pl_.N, o, c); regrow(pl_.sP, o, c); regrow(pl_.sX0, o, c);
+        regrow(pl_.rest4, o, c); regrow(pl_.brest4, o, c); regrow(pl_.k0, o, c); regrow(pl_.k1, o, c);
+        regrow(pl_.v0, o, c); regrow(pl_.v1, o, c); regrow(pl_.gflags, o, c);
+    });
+    grow(acell_, need.n_cell, [&](size_t o, size_t c) { regrow(pl_.cells, o, c); });
+    grow(anbr_, need.n_nbr, [&](size_t o, size_t c) { regrow(pl_.nbr, o, c); });
+    grow(ast_, need.n_st, [&](size_t o, size_t c) { regrow(pl_.st, o, c); });
+    grow(aat_, need.n_at, [&](size_t o, size_t c) { regrow(pl_.at, o, c); });
+    grow(abd_, need.n_bd, [&](size_t o, size_t c) { regrow(pl_.bd, o, c); });
+    grow(atr_, need.n_tr, [&](size_t o, size_t c) { regrow(pl_.tr, o, c); });
+    grow(abrec_, need.n_brec, [&](size_t o, size_t c) { regrow(pl_.brec, o, c); regrow(pl_.brest, o, c); });
+    grow(atrec_, need.n_trec, [&](size_t o, size_t c) { regrow(pl_.trec, o, c); });
+    grow(aslot_, need.n_slot, [&](size_t o, size_t c) { regrow(pl_.slots, o, c); });
+    grow(acol_, need.n_col, [&](size_t o, size_t c) { regrow(pl_.cols, o, c); });
+}
+
+void Engine::free_pools() {
+    void* ptrs[] = {pl_.X, pl_.V, pl_.P0, pl_.P1, pl_.X0, pl_.N, pl_.sP, pl_.sX0, pl_.rest4, pl_.brest4,
+                    pl_.k0, pl_.k1, pl_.v0, pl_.v1, pl_.gflags, pl_.cells, pl_.nbr, pl_.st, pl_.at, pl_.bd,
+                    pl_.tr, pl_.brec, pl_.trec, pl_.brest, pl_.slots, pl_.cols};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    pl_ = Pools{};
+}
+
+template <class T>
+static void h2d(T* dst, const std::vector<T>& src, hipStream_t s) {
+    if (!src.empty()) VTX_HIP(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
+}
+
+// Copy one prepared segment into freshly taken pool ranges.
+void Engine::upload(Segment& sg) {
+    const Ranges need = needs_of(sg);
+    grow_pools(need);
+    Ranges r = need;
+    auto take = [](RangeAlloc& a, size_t n, size_t* off) {
+        if (!a.take(n, off)) throw Error(VTX_ENOMEM, "pool range allocation failed");
+    };
+    take(ap_, need.n_p, &r.p);
+    take(acell_, need.n_cell, &r.cell);
+    take(anbr_, need.n_nbr, &r.nbr);
+    take(ast_, need.n_st, &r.st);
+    take(aat_, need.n_at, &r.at);
+    take(abd_, need.n_bd, &r.bd);
+    take(atr_, need.n_tr, &r.tr);
+    take(abrec_, need.n_brec, &r.brec);
+    take(atrec_, need.n_trec, &r.trec);
+    take(aslot_, need.n_slot, &r.slot);
+    take(acol_, need.n_col, &r.col);
+    sg.rg = r;
+
+    const Prepared& pr = sg.prep;
+    const vtx_params& p = sg.prm;
+    const uint32_t n = sg.n();
+    SegDev d{};
+    d.pbase = (uint32_t)r.p;
+    d.n = n;
+    const bool selfcol = sg.selfcol();
+    d.M = selfcol ? sg.table_size() : 0;
+    VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
+    d.tbase = (uint32_t)r.cell;
+    d.nbase_lo = (uint32_t)(r.nbr & 0xFFFFFFFFu);
+    d.nbase_hi = (uint32_t)((uint64_t)r.nbr >> 32);
+    d.sbase = (uint32_t)r.slot;
+    d.scount = (uint32_t)pr.slots.size();
+    d.cbase = (uint32_t)r.col;
+    d.ccount = (uint32_t)pr.cols.size();
+    d.bbase = (uint32_t)r.brec;
+    d.trbase = (uint32_t)r.trec;
+    d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u);
+    d.max_nbr = (uint32_t)p.max_num_neighbors;
+    // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
+    d.diameter = p.particle_diameter;
+    d.cell = p.particle_diameter * p.hash_cell_size_scalar;
+    d.cell2 = d.cell * d.cell;
+    d.diam2 = p.particle_diameter * p.particle_diameter;
+    d.max_speed = p.max_speed;
+    d.damping = p.damping;
+    d.omega = p.relaxation_factor;
+    d.lrs = p.long_range_stretchiness;
+    d.bend_compliance = p.bend_compliance;
+    d.margin = p.collision_margin;
+    d.friction = p.friction;
+    d.stiffness = p.stretch_stiffness;
+    d.gx = p.gravity[0];
+    d.gy = p.gravity[1];
+    d.gz = p.gravity[2];
+    if (selfcol) {
+        const uint64_t c = UINT64_MAX / d.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
+        d.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
+        d.mod_hi = (uint32_t)(c >> 32);
+    }
+    d.rows = pr.grows;
+    d.cols = pr.gcols;
+    sg.dev = d;
+
+    hipStream_t s = stream_;
+    h2d(pl_.X + r.p, pr.x, s);
+    h2d(pl_.V + r.p, pr.v, s);
+    h2d(pl_.P0 + r.p, pr.x, s);
+    h2d(pl_.P1 + r.p, pr.x, s);
+    h2d(pl_.X0 + r.p, pr.x0, s);
+    if (n) VTX_HIP(hipMemsetAsync(pl_.N + r.p, 0, sizeof(float4) * n, s));
+    if (pr.grid) {
+        h2d(pl_.rest4 + r.p, pr.rest4, s);
+        h2d(pl_.gflags + r.p, pr.gflags, s);
+        h2d(pl_.brest4 + r.p, pr.brest4, s);
+    }
+    if (r.n_cell) VTX_HIP(hipMemsetAsync(pl_.cells + r.cell, 0xFF, sizeof(uint2) * r.n_cell, s));
+    if (r.n_nbr) VTX_HIP(hipMemsetAsync(pl_.nbr + r.nbr, 0xFF, sizeof(uint32_t) * r.n_nbr, s));
+    h2d(pl_.st + r.st, pr.est, s);
+    h2d(pl_.at + r.at, pr.eat, s);
+    h2d(pl_.bd + r.bd, pr.ebd, s);
+    h2d(pl_.tr + r.tr, pr.etr, s);
+    h2d(pl_.brec + r.brec, pr.brec, s);
+    h2d(pl_.brest + r.brec, pr.brest, s);
+    h2d(pl_.trec + r.trec, pr.trec, s);
+    h2d(pl_.slots + r.slot, pr.slots, s);
+    h2d(pl_.cols + r.col, pr.cols, s);
+    VTX_HIP(hipStreamSynchronize(s));
+    sg.uploaded = true;
+}
+
+void Engine::release(Segment& sg) {
+    if (!sg.uploaded) return;
+    const Ranges& r = sg.rg;
+    ap_.give(r.p, r.n_p);
+    acell_.give(r.cell, r.n_cell);
+    anbr_.give(r.nbr, r.n_nbr);
+    ast_.give(r.st, r.n_st);
+    aat_.give(r.at, r.n_at);
+    abd_.give(r.bd, r.n_bd);
+    atr_.give(r.tr, r.n_tr);
+    abrec_.give(r.brec, r.n_brec);
+    atrec_.give(r.trec, r.n_trec);
+    aslot_.give(r.slot, r.n_slot);
+    acol_.give(r.col, r.n_col);
+    sg.uploaded = false;
+}
+
+void Engine::free_tables() {
+    for (void* p : table_allocs_) (void)hipFree(p);
+    table_allocs_.clear();
+}
+
+template <class T>
+static T* table_upload(std::vector<void*>& allocs, const std::vector<T>& h) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(sizeof(T) * h.size(), 256);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(VTX_ENOMEM, "hipMalloc of a table failed");
+    }
+    allocs.push_back(p);
+    if (!h.empty()) VTX_HIP(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return (T*)p;
+}
+
+static uint32_t ceil_log2(uint32_t v) {
+    uint32_t b = 0;
+    while ((1ull << b) < v) b++;
+    return b;
+}
+
+// Segment, tile and sort tables of the active segments.
+void Engine::rebuild_tables() {
+    VTX_HIP(hipStreamSynchronize(stream_));
+    invalidate_graph();
+    free_tables();
+    std::vector<SegDev> hseg;
+    std::vector<TileDev> htile, hitile;
+    std::vector<GridTileDev> hgtile;
+    std::vector<SortTileDev> hst;
+    table_seg_.clear();
+    any_selfcol_ = false;
+    need_collide_ = false;
+    live_particles_ = 0;
+    uint32_t maxbits = 0, hbase = 0, compact = 0;
+    bool first = true;
+    for (size_t si = 0; si < segs_.size(); si++) {
+        Segment* sgp = segs_[si].get();
+        if (!sgp || !sgp->active || !sgp->uploaded || sgp->n() == 0) continue;
+        Segment& sg = *sgp;
+        const vtx_params& p = sg.prm;
+        if (first) {
+            substeps_ = p.num_substeps;
+            iterations_ = p.num_iterations;
+            interleave_ = p.interleaved_hash;
+            first = false;
+        }
+        VTX_REQUIRE(p.num_substeps == substeps_ && p.num_iterations == iterations_ && p.interleaved_hash == interleave_,
+                    VTX_EINVAL, "active segments of one solver must share substeps, iterations and interleave");
+        const uint32_t ti = (uint32_t)hseg.size();
+        table_seg_.push_back((int32_t)si);
+        hseg.push_back(sg.dev);
+        const SegDev& d = sg.dev;
+        live_particles_ += d.n;
+        if (d.flags & VTX_SEG_SELFCOL) {
+            any_selfcol_ = true;
+            maxbits = std::max(maxbits, std::max(1u, ceil_log2(d.M)));
+        }
+        if ((d.flags & VTX_SEG_SELFCOL) || d.ccount) need_collide_ = true;
+        for (TileDev t : sg.prep.tiles) {
+            t.seg = ti;
+            t.begin += d.pbase;
+            t.st_off += (uint32_t)sg.rg.st;
+            t.at_off += (uint32_t)sg.rg.at;
+            t.bd_off += (uint32_t)sg.rg.bd;
+            t.tr_off += (uint32_t)sg.rg.tr;
+            htile.push_back(t);
+            if (!sg.prep.grid) hitile.push_back(t);
+        }
+        for (GridTileDev gt : sg.prep.gtiles) {
+            gt.seg = ti;
+            gt.at_off += (uint32_t)sg.rg.at;
+            hgtile.push_back(gt);
+        }
+        const uint32_t nst = (d.n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
+        for (uint32_t k = 0; k < nst; k++) {
+            SortTileDev st{};
+            st.seg = ti;
+            st.begin = d.pbase + k * VTX_SORT_TILE;
+            st.count = std::min<uint32_t>(VTX_SORT_TILE, d.n - k * VTX_SORT_TILE);
+            st.hbase = hbase;
+            st.ntiles = nst;
+            st.tidx = k;
+            st.bias = d.pbase - compact;   // scan positions are compact; outputs go to the segment's range
+            hst.push_back(st);
+        }
+        hbase += 256u * nst;
+        compact += d.n;
+    }
+    passes_ = std::max(1, (int)((maxbits + 7) / 8));
+    VTX_REQUIRE((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK <= 65536u, VTX_ECAPACITY, "histogram too large");
+    const size_t npart = (hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1;
+    if (hbase > hist_cap_) {
+        if (hist_) (void)hipFree(hist_);
+        hist_cap_ = hbase + hbase / 2;
+        VTX_HIP(hipMalloc((void**)&hist_, sizeof(uint32_t) * std::max<size_t>(hist_cap_, 64)));
+    }
+    if (npart > part_cap_) {
+        if (part_) (void)hipFree(part_);
+        part_cap_ = npart + npart / 2;
+        VTX_HIP(hipMalloc((void**)&part_, sizeof(uint32_t) * std::max<size_t>(part_cap_, 64)));
+    }
+    if (hseg.size() > flag_cap_) {
+        if (flags_) (void)hipFree(flags_);
+        flag_cap_ = hseg.size() + hseg.size() / 2 + 16;
+        VTX_HIP(hipMalloc((void**)&flags_, sizeof(uint32_t) * flag_cap_));
+    }
+
+    DevState d{};
+    d.nparticles = compact;
+    d.ntiles = (uint32_t)htile.size();
+    d.nitiles = (uint32_t)hitile.size();
+    d.ngtiles = (uint32_t)hgtile.size();
+    d.nstiles = (uint32_t)hst.size();
+    d.nhist = hbase;
+    d.ncells = (uint32_t)acell_.cap;
+    d.segs = table_upload(table_allocs_, hseg);
+    d.tiles = table_upload(table_allocs_, htile);
+    d.itiles = table_upload(table_allocs_, hitile);
+    d.gtiles = table_upload(table_allocs_, hgtile);
+    d.stiles = table_upload(table_allocs_, hst);
+    d.X = pl_.X;
+    d.V = pl_.V;
+    d.P[0] = pl_.P0;
+    d.P[1] = pl_.P1;
+    d.X0 = pl_.X0;
+    d.Nrm = pl_.N;
+    d.sP = pl_.sP;
+    d.sX0 = pl_.sX0;
+    d.rest4 = pl_.rest4;
+    d.brest4 = pl_.brest4;
+    d.gflags = pl_.gflags;
+    d.keys[0] = pl_.k0;
+    d.keys[1] = pl_.k1;
+    d.vals[0] = pl_.v0;
+    d.vals[1] = pl_.v1;
+    d.cells = pl_.cells;
+    d.nbr = pl_.nbr;
+    d.st = pl_.st;
+    d.at = pl_.at;
+    d.bd = pl_.bd;
+    d.tr = pl_.tr;
+    d.brec = pl_.brec;
+    d.brest = pl_.brest;
+    d.trec = pl_.trec;
+    d.slots = pl_.slots;
+    d.cols = pl_.cols;
+    d.hist = hist_;
+    d.part = part_;
+    ds_ = d;
+    tables_dirty_ = false;
+}
+
+void Engine::ensure_committed() {
+    // size the pools once for everything that is about to be uploaded
+    Ranges need;
+    bool any = false;
+    for (size_t si = 0; si < segs_.size(); si++) {
+        Segment* sg = segs_[si].get();
+        if (!sg || !sg->active || sg->uploaded) continue;
+        prepare((int32_t)si);
+        const Ranges r = needs_of(*sg);
+        need.n_p += r.n_p; need.n_cell += r.n_cell; need.n_nbr += r.n_nbr; need.n_st += r.n_st;
+        need.n_at += r.n_at; need.n_bd += r.n_bd; need.n_tr += r.n_tr; need.n_brec += r.n_brec;
+        need.n_trec += r.n_trec; need.n_slot += r.n_slot; need.n_col += r.n_col;
+        any = true;
+    }
+    if (any) {
+        VTX_HIP(hipStreamSynchronize(stream_));
+        grow_pools(need);
+        for (auto& sg : segs_)
+            if (sg && sg->active && !sg->uploaded) upload(*sg);
+        tables_dirty_ = true;
+    }
+    if (tables_dirty_) rebuild_tables();
+}
+
+// ---------------------------------------------------------------------------- grid path switch
 void Engine::set_stencil_path(bool on) {
     if (on == use_stencil_) return;
-    pull_state();
+    for (size_t si = 0; si < segs_.size(); si++)
+        if (segs_[si]) touch((int32_t)si);
     use_stencil_ = on;
-    mark_dirty();
 }
 
 int32_t Engine::stencil_segments() {
     ensure_committed();
     int32_t k = 0;
-    for (uint8_t g : seg_grid_) k += g;
+    for (int32_t si : table_seg_) k += segs_[(size_t)si]->prep.grid ? 1 : 0;
     return k;
-}
-
-void Engine::commit() {
-    VTX_HIP(hipStreamSynchronize(stream_));
-    if (gexec_) {
-        hipGraphExecDestroy(gexec_);
-        gexec_ = nullptr;
-    }
-    free_device();
-    hseg_.clear();
-    htile_.clear();
-
-    const vtx_params& p0 = segs_[0].prm;
-    substeps_ = p0.num_substeps;
-    iterations_ = p0.num_iterations;
-    interleave_ = p0.interleaved_hash;
-    for (auto& sg : segs_) {
-        VTX_REQUIRE(sg.prm.num_substeps == substeps_ && sg.prm.num_iterations == iterations_ &&
-                        sg.prm.interleaved_hash == interleave_ &&
-                        sg.prm.max_num_neighbors == p0.max_num_neighbors,
-                    VTX_EINVAL, "segments of one solver must share substeps, iterations, interleave and max neighbours");
-    }
-
-    std::vector<float4> hx, hv, hx0;
-    std::vector<uint2> est, eat;
-    std::vector<uint32_t> ebd, etr;
-    std::vector<uint4> brec, trec;
-    std::vector<float> brest;
-    std::vector<float4> hslots;
-    std::vector<ColliderDev> hcols;
-    std::vector<SortTileDev> hst;
-    std::vector<TileDev> hitile;
-    std::vector<GridTileDev> hgtile;
-    std::vector<float4> hrest4, hbrest4;
-    std::vector<uint32_t> hgflags;
-    seg_grid_.assign(segs_.size(), 0);
-
-    uint32_t pbase = 0, tbase = 0, sbase = 0, cbase = 0, bbase = 0, trbase = 0, hbase = 0;
-    uint64_t nbase = 0;
-    uint32_t maxbits = 0;
-    any_selfcol_ = false;
-    need_collide_ = false;
-
-    for (size_t si = 0; si < segs_.size(); si++) {
-        const Segment& sg = segs_[si];
-        const vtx_params& p = sg.prm;
-        const uint32_t n = sg.n();
-        SegDev d{};
-        d.pbase = pbase;
-        d.n = n;
-        const bool selfcol = p.enable_self_collision && n > 0;
-        d.M = selfcol ? sg.table_size() : 0;
-        VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
-        VTX_REQUIRE(!selfcol || n < (1u << 25), VTX_ECAPACITY,
-                    "self collision supports at most 2^25 particles per segment");
-        d.tbase = tbase;
-        d.nbase_lo = (uint32_t)(nbase & 0xFFFFFFFFu);
-        d.nbase_hi = (uint32_t)(nbase >> 32);
-        d.sbase = sbase;
-        d.scount = (uint32_t)(sg.slots.size() / 3);
-        d.cbase = cbase;
-        d.ccount = (uint32_t)sg.cols.size();
-        d.flags = selfcol ? VTX_SEG_SELFCOL : 0u;
-        d.max_nbr = (uint32_t)p.max_num_neighbors;
-        // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
-        d.diameter = p.particle_diameter;
-        d.cell = p.particle_diameter * p.hash_cell_size_scalar;
-        d.cell2 = d.cell * d.cell;
-        d.diam2 = p.particle_diameter * p.particle_diameter;
-        d.max_speed = p.max_speed;
-        d.damping = p.damping;
-        d.omega = p.relaxation_factor;
-        d.lrs = p.long_range_stretchiness;
-        d.bend_compliance = p.bend_compliance;
-        d.margin = p.collision_margin;
-        d.friction = p.friction;
-        d.stiffness = p.stretch_stiffness;
-        d.gx = p.gravity[0];
-        d.gy = p.gravity[1];
-        d.gz = p.gravity[2];
-        if (selfcol) {
-            const uint64_t c = UINT64_MAX / d.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
-            d.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
-            d.mod_hi = (uint32_t)(c >> 32);
-        }
-        GridLayout G;
-        const bool grid = use_stencil_ && detect_grid(sg, G);
-        seg_grid_[si] = grid ? 1 : 0;
-        if (grid) {
-            d.flags |= VTX_SEG_GRID | (G.rest0 ? VTX_SEG_BEND_REST0 : 0u);
-            // per slot type: all geometrically possible slots present, or none
-            for (int type = 0; type < 3; type++) {
-                bool full = true, none = true;
-                for (uint32_t a = 0; a < G.rows && (full || none); a++)
-                    for (uint32_t b = 0; b < G.cols; b++) {
-                        const bool possible = a + 1 < G.rows && b + 1 < G.cols && (type != 1 || a >= 1) &&
-                                              (type != 2 || b >= 1);
-                        const bool present = (G.flags[(size_t)a * G.cols + b] >> type) & 1u;
-                        if (present != possible) full = false;
-                        if (present) none = false;
-                    }
-                if (full) d.flags |= VTX_SEG_FULL_X << type;
-                if (none) d.flags |= VTX_SEG_NONE_X << type;
-            }
-            d.rows = G.rows;
-            d.cols = G.cols;
-            hrest4.insert(hrest4.end(), G.rest4.begin(), G.rest4.end());
-            hgflags.insert(hgflags.end(), G.flags.begin(), G.flags.end());
-            hbrest4.insert(hbrest4.end(), G.brest4.begin(), G.brest4.end());
-        } else {
-            hrest4.resize(hrest4.size() + n, make_float4(0.f, 0.f, 0.f, 0.f));
-            hgflags.resize(hgflags.size() + n, 0u);
-            hbrest4.resize(hbrest4.size() + n, make_float4(0.f, 0.f, 0.f, 0.f));
-        }
-        hseg_.push_back(d);
-        if (selfcol) {
-            any_selfcol_ = true;
-            maxbits = std::max(maxbits, std::max(1u, ceil_log2(d.M)));
-        }
-        if (selfcol || d.ccount) need_collide_ = true;
-
-        for (uint32_t i = 0; i < n; i++) {
-            hx.push_back(make_float4(sg.x[3 * i], sg.x[3 * i + 1], sg.x[3 * i + 2], sg.w[i]));
-            hv.push_back(make_float4(sg.v[3 * i], sg.v[3 * i + 1], sg.v[3 * i + 2], 0.f));
-            hx0.push_back(make_float4(sg.x0[3 * i], sg.x0[3 * i + 1], sg.x0[3 * i + 2], 0.f));
-        }
-        for (uint32_t k = 0; k < d.scount; k++)
-            hslots.push_back(make_float4(sg.slots[3 * k], sg.slots[3 * k + 1], sg.slots[3 * k + 2], 0.f));
-        for (auto& c : sg.cols) hcols.push_back(pack_collider(c));
-
-        // incidence lists (CSR over local particles, registration order)
-        std::vector<uint32_t> s_st(n + 1, 0), s_at(n + 1, 0), s_bd(n + 1, 0), s_tr(n + 1, 0);
-        const size_t S = sg.st_rest.size(), A = sg.at_pid.size(), B = sg.bd_rest.size(), T = sg.tris.size() / 3;
-        for (size_t c = 0; c < S; c++) {
-            s_st[(size_t)sg.st[2 * c] + 1]++;
-            s_st[(size_t)sg.st[2 * c + 1] + 1]++;
-        }
-        for (size_t c = 0; c < A; c++) s_at[(size_t)sg.at_pid[c] + 1]++;
-        for (size_t c = 0; c < B; c++)
-            for (int r = 0; r < 4; r++) s_bd[(size_t)sg.bd[4 * c + r] + 1]++;
-        for (size_t c = 0; c < T; c++)
-            for (int r = 0; r < 3; r++) s_tr[(size_t)sg.tris[3 * c + r] + 1]++;
-        for (uint32_t i = 0; i < n; i++) {
-            s_st[i + 1] += s_st[i];
-            s_at[i + 1] += s_at[i];
-            s_bd[i + 1] += s_bd[i];
-            s_tr[i + 1] += s_tr[i];
-        }
-        std::vector<uint2> l_st(s_st[n]), l_at(s_at[n]);
-        std::vector<uint32_t> l_bd(s_bd[n]), l_tr(s_tr[n]);
-        {
-            std::vector<uint32_t> f(s_st.begin(), s_st.end() - 1);
-            for (size_t c = 0; c < S; c++) {
-                const uint32_t a = (uint32_t)sg.st[2 * c], b = (uint32_t)sg.st[2 * c + 1];
-                const uint32_t r = fbits(sg.st_rest[c]);
-                l_st[f[a]++] = make_uint2(pbase + b, r);
-                l_st[f[b]++] = make_uint2(pbase + a, r);
-            }
-        }
-        {
-            std::vector<uint32_t> f(s_at.begin(), s_at.end() - 1);
-            for (size_t c = 0; c < A; c++)
-                l_at[f[(size_t)sg.at_pid[c]]++] = make_uint2(sbase + (uint32_t)sg.at_slot[c], fbits(sg.at_dist[c]));
-        }
-        {
-            std::vector<uint32_t> f(s_bd.begin(), s_bd.end() - 1);
-            for (size_t c = 0; c < B; c++)
-                for (uint32_t r = 0; r < 4; r++)
-                    l_bd[f[sg.bd[4 * c + r]]++] = (bbase + (uint32_t)c) | (r << 30);
-        }
-        {
-            std::vector<uint32_t> f(s_tr.begin(), s_tr.end() - 1);
-            for (size_t c = 0; c < T; c++)
-                for (int r = 0; r < 3; r++) l_tr[f[sg.tris[3 * c + r]]++] = trbase + (uint32_t)c;
-        }
-        for (size_t c = 0; c < B; c++) {
-            brec.push_back(make_uint4(pbase + sg.bd[4 * c], pbase + sg.bd[4 * c + 1], pbase + sg.bd[4 * c + 2],
-                                      pbase + sg.bd[4 * c + 3]));
-            brest.push_back(sg.bd_rest[c]);
-        }
-        for (size_t c = 0; c < T; c++)
-            trec.push_back(make_uint4(pbase + sg.tris[3 * c], pbase + sg.tris[3 * c + 1], pbase + sg.tris[3 * c + 2], 0));
-
-        for (uint32_t first = 0; first < n; first += VTX_TILE) {
-            TileDev t{};
-            t.seg = (uint32_t)si;
-            t.begin = pbase + first;
-            t.count = std::min<uint32_t>(VTX_TILE, n - first);
-            if (!grid) {   // the stencil path keeps its own attach lists and needs no stretch/bend lists
-                pack_tile(est, s_st, l_st, first, t.count, make_uint2(VTX_INVALID, 0u), &t.st_off, &t.st_k);
-                pack_tile(eat, s_at, l_at, first, t.count, make_uint2(VTX_INVALID, 0u), &t.at_off, &t.at_k);
-                pack_tile(ebd, s_bd, l_bd, first, t.count, VTX_INVALID, &t.bd_off, &t.bd_k);
-            }
-            pack_tile(etr, s_tr, l_tr, first, t.count, VTX_INVALID, &t.tr_off, &t.tr_k);
-            htile_.push_back(t);
-            if (!grid) hitile.push_back(t);
-        }
-        if (grid) {
-            for (uint32_t a0 = 0; a0 < G.rows; a0 += VTX_GT)
-                for (uint32_t b0 = 0; b0 < G.cols; b0 += VTX_GT) {
-                    GridTileDev gt{};
-                    gt.seg = (uint32_t)si;
-                    gt.a0 = a0;
-                    gt.b0 = b0;
-                    std::vector<int64_t> lanes(VTX_TILE, -1);
-                    for (uint32_t l = 0; l < VTX_TILE; l++) {
-                        const uint32_t a = a0 + l / VTX_GT, b = b0 + l % VTX_GT;
-                        if (a < G.rows && b < G.cols) lanes[l] = (int64_t)a * G.cols + b;
-                    }
-                    pack_lanes(eat, s_at, l_at, lanes, make_uint2(VTX_INVALID, 0u), &gt.at_off, &gt.at_k);
-                    hgtile.push_back(gt);
-                }
-        }
-        const uint32_t nst = (n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
-        for (uint32_t k = 0; k < nst; k++) {
-            SortTileDev st{};
-            st.seg = (uint32_t)si;
-            st.begin = pbase + k * VTX_SORT_TILE;
-            st.count = std::min<uint32_t>(VTX_SORT_TILE, n - k * VTX_SORT_TILE);
-            st.hbase = hbase;
-            st.ntiles = nst;
-            st.tidx = k;
-            hst.push_back(st);
-        }
-        hbase += 256u * nst;
-        pbase += n;
-        tbase += d.M;
-        nbase += selfcol ? (uint64_t)d.max_nbr * n : 0;
-        sbase += d.scount;
-        cbase += d.ccount;
-        bbase += (uint32_t)B;
-        trbase += (uint32_t)T;
-    }
-    passes_ = std::max(1, (int)((maxbits + 7) / 8));
-
-    DevState d{};
-    d.nparticles = pbase;
-    d.ntiles = (uint32_t)htile_.size();
-    d.nitiles = (uint32_t)hitile.size();
-    d.ngtiles = (uint32_t)hgtile.size();
-    d.itiles = dupload(hitile);
-    d.gtiles = dupload(hgtile);
-    d.rest4 = dupload(hrest4);
-    d.gflags = dupload(hgflags);
-    d.brest4 = dupload(hbrest4);
-    d.nstiles = (uint32_t)hst.size();
-    d.nhist = hbase;
-    d.ncells = tbase;
-    d.segs = dupload(hseg_);
-    d.tiles = dupload(htile_);
-    d.stiles = dupload(hst);
-    d.X = dupload(hx);
-    d.V = dupload(hv);
-    d.X0 = dupload(hx0);
-    d.P[0] = dupload(hx);
-    d.P[1] = dupload(hx);
-    d.Nrm = dalloc<float4>(pbase);
-    VTX_HIP(hipMemset(d.Nrm, 0, sizeof(float4) * std::max<uint32_t>(pbase, 1)));
-    d.st = dupload(est);
-    d.at = dupload(eat);
-    d.bd = dupload(ebd);
-    d.brec = dupload(brec);
-    d.brest = dupload(brest);
-    d.tr = dupload(etr);
-    d.trec = dupload(trec);
-    d.slots = dupload(hslots);
-    d.cols = dupload(hcols);
-    if (any_selfcol_) {
-        for (int k = 0; k < 2; k++) {
-            d.keys[k] = dalloc<uint32_t>(pbase);
-            d.vals[k] = dalloc<uint32_t>(pbase);
-        }
-        d.hist = dalloc<uint32_t>(hbase);
-        d.part = dalloc<uint32_t>((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1);
-        d.cells = dalloc<uint2>(tbase);
-        VTX_HIP(hipMemset(d.cells, 0xFF, sizeof(uint2) * std::max<uint32_t>(tbase, 1)));
-        d.sP = dalloc<float4>(pbase);
-        d.sX0 = dalloc<float4>(pbase);
-        d.nbr = dalloc<uint32_t>((size_t)nbase);
-        VTX_HIP(hipMemset(d.nbr, 0xFF, sizeof(uint32_t) * std::max<uint64_t>(nbase, 1)));
-    }
-    VTX_REQUIRE((d.nhist + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK <= 65536u, VTX_ECAPACITY, "histogram too large");
-    VTX_HIP(hipDeviceSynchronize());
-    ds_ = d;
-    dirty_ = false;
-}
-
-void Engine::ensure_committed() {
-    if (dirty_) commit();
 }
 
 // ---------------------------------------------------------------------------- frames
@@ -783,7 +1060,7 @@ void Engine::hash_build(int in, hipStream_t s) {
     }
     stage_end(s);
     stage_begin("hash_cells", s);
-    VTX_HIP(launch_fill_u32((uint32_t*)ds_.cells, 2 * ds_.ncells, VTX_INVALID, s));
+    VTX_HIP(launch_clear_cells(ds_, s));
     VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
@@ -847,22 +1124,19 @@ void Engine::simulate(float frame_dt) {
         }
     } else if (use_graph_) {
         if (!gexec_ || graph_dt_ != frame_dt) {
-            if (gexec_) {
-                hipGraphExecDestroy(gexec_);
-                gexec_ = nullptr;
-            }
+            invalidate_graph();
             hipGraph_t graph = nullptr;
             VTX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             try {
                 record_frame(frame_dt, stream_);
             } catch (...) {
-                hipStreamEndCapture(stream_, &graph);
-                if (graph) hipGraphDestroy(graph);
+                (void)hipStreamEndCapture(stream_, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
                 throw;
             }
             VTX_HIP(hipStreamEndCapture(stream_, &graph));
             hipError_t e = hipGraphInstantiate(&gexec_, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             VTX_HIP(e);
             graph_dt_ = frame_dt;
         }
@@ -870,22 +1144,18 @@ void Engine::simulate(float frame_dt) {
     } else {
         record_frame(frame_dt, stream_);
     }
-    dev_live_ = true;
 }
 
 void Engine::synchronize() { VTX_HIP(hipStreamSynchronize(stream_)); }
 
 void Engine::set_use_graph(bool on) {
     use_graph_ = on;
-    if (!on && gexec_) {
-        hipGraphExecDestroy(gexec_);
-        gexec_ = nullptr;
-    }
+    if (!on) invalidate_graph();
 }
 
 void Engine::set_stream(hipStream_t s) {
     synchronize();
-    if (own_stream_ && stream_) hipStreamDestroy(stream_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     if (s) {
         stream_ = s;
         own_stream_ = false;
@@ -893,10 +1163,7 @@ void Engine::set_stream(hipStream_t s) {
         VTX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         own_stream_ = true;
     }
-    if (gexec_) {
-        hipGraphExecDestroy(gexec_);
-        gexec_ = nullptr;
-    }
+    invalidate_graph();
 }
 
 void Engine::set_timing(bool on) {
@@ -921,16 +1188,25 @@ std::string Engine::timing_report() {
 
 // ---------------------------------------------------------------------------- state access
 void Engine::read_particles(int32_t s, int what, float* out) {
-    check_seg(s);
+    Segment& sg = seg(s);
     VTX_REQUIRE(out, VTX_EINVAL, "output is NULL");
     ensure_committed();
+    const uint32_t n = sg.n();
+    if (!n) return;
+    if (!sg.uploaded) {   // pending segment: host state
+        for (uint32_t i = 0; i < n; i++) {
+            if (what == 3) out[i] = sg.w[i];
+            else
+                for (int k = 0; k < 3; k++)
+                    out[3 * i + k] = what == 0 ? sg.x[3 * i + k] : what == 1 ? sg.v[3 * i + k] : 0.f;
+        }
+        return;
+    }
     synchronize();
-    const SegDev& d = hseg_[(size_t)s];
-    if (!d.n) return;
-    std::vector<float4> tmp(d.n);
-    const float4* src = what == 0 ? ds_.X : what == 1 ? ds_.V : what == 2 ? ds_.Nrm : ds_.X;
-    VTX_HIP(hipMemcpy(tmp.data(), src + d.pbase, sizeof(float4) * d.n, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < d.n; i++) {
+    std::vector<float4> tmp(n);
+    const float4* src = what == 0 ? pl_.X : what == 1 ? pl_.V : what == 2 ? pl_.N : pl_.X;
+    VTX_HIP(hipMemcpy(tmp.data(), src + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
         if (what == 3) {
             out[i] = tmp[i].w;
         } else {
@@ -942,97 +1218,115 @@ void Engine::read_particles(int32_t s, int what, float* out) {
 }
 
 float4* Engine::device_positions(int32_t s, uint32_t* n) {
-    check_seg(s);
+    Segment& sg = seg(s);
     ensure_committed();
-    if (n) *n = hseg_[(size_t)s].n;
-    return ds_.X + hseg_[(size_t)s].pbase;
+    VTX_REQUIRE(sg.uploaded, VTX_ESTATE, "segment is not resident on the device");
+    if (n) *n = sg.n();
+    return pl_.X + sg.rg.p;
 }
 
 float4* Engine::device_normals(int32_t s, uint32_t* n) {
-    check_seg(s);
+    Segment& sg = seg(s);
     ensure_committed();
-    if (n) *n = hseg_[(size_t)s].n;
-    return ds_.Nrm + hseg_[(size_t)s].pbase;
+    VTX_REQUIRE(sg.uploaded, VTX_ESTATE, "segment is not resident on the device");
+    if (n) *n = sg.n();
+    return pl_.N + sg.rg.p;
 }
 
 // MouseGrabber-style edit (Velvet/MouseGrabber.hpp:51-52,60,74-75).
 void Engine::set_particle(int32_t s, int32_t i, const float* pos, const float* vel, float inv_mass) {
     Segment& sg = seg(s);
     VTX_REQUIRE(i >= 0 && (uint32_t)i < sg.n(), VTX_EINVAL, "particle index out of range");
-    if (dirty_ || !dev_live_) {
+    if (!sg.uploaded) {
         if (pos) std::copy(pos, pos + 3, sg.x.begin() + 3 * i);
         if (vel) std::copy(vel, vel + 3, sg.v.begin() + 3 * i);
         if (inv_mass >= 0.f) sg.w[(size_t)i] = inv_mass;
-        if (!dirty_) mark_dirty();
+        sg.prepared = false;
         return;
     }
     synchronize();
-    const uint32_t g = hseg_[(size_t)s].pbase + (uint32_t)i;
+    const size_t g = sg.rg.p + (size_t)i;
     float4 x;
-    VTX_HIP(hipMemcpy(&x, ds_.X + g, sizeof(float4), hipMemcpyDeviceToHost));
+    VTX_HIP(hipMemcpy(&x, pl_.X + g, sizeof(float4), hipMemcpyDeviceToHost));
     if (pos) {
         x.x = pos[0];
         x.y = pos[1];
         x.z = pos[2];
     }
     if (inv_mass >= 0.f) x.w = inv_mass;
-    VTX_HIP(hipMemcpy(ds_.X + g, &x, sizeof(float4), hipMemcpyHostToDevice));
+    VTX_HIP(hipMemcpy(pl_.X + g, &x, sizeof(float4), hipMemcpyHostToDevice));
     if (vel) {
-        float4 v = make_float4(vel[0], vel[1], vel[2], 0.f);
-        VTX_HIP(hipMemcpy(ds_.V + g, &v, sizeof(float4), hipMemcpyHostToDevice));
+        const float4 v = make_float4(vel[0], vel[1], vel[2], 0.f);
+        VTX_HIP(hipMemcpy(pl_.V + g, &v, sizeof(float4), hipMemcpyHostToDevice));
     }
 }
 
 void Engine::write_predicted(int32_t s, const float* xyz) {
-    check_seg(s);
+    Segment& sg = seg(s);
     VTX_REQUIRE(xyz, VTX_EINVAL, "positions are NULL");
     ensure_committed();
+    VTX_REQUIRE(sg.uploaded, VTX_ESTATE, "segment is not resident on the device");
     synchronize();
-    const SegDev& d = hseg_[(size_t)s];
-    if (!d.n) return;
-    std::vector<float4> tmp(d.n);
-    VTX_HIP(hipMemcpy(tmp.data(), ds_.X + d.pbase, sizeof(float4) * d.n, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < d.n; i++) {
+    const uint32_t n = sg.n();
+    if (!n) return;
+    std::vector<float4> tmp(n);
+    VTX_HIP(hipMemcpy(tmp.data(), pl_.X + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
         tmp[i].x = xyz[3 * i];
         tmp[i].y = xyz[3 * i + 1];
         tmp[i].z = xyz[3 * i + 2];
     }
-    VTX_HIP(hipMemcpy(ds_.P[0] + d.pbase, tmp.data(), sizeof(float4) * d.n, hipMemcpyHostToDevice));
+    VTX_HIP(hipMemcpy(pl_.P0 + sg.rg.p, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
 }
 
 void Engine::run_hash() {
     ensure_committed();
-    VTX_REQUIRE(any_selfcol_, VTX_ESTATE, "no segment has self collision enabled");
+    VTX_REQUIRE(any_selfcol_, VTX_ESTATE, "no active segment has self collision enabled");
     hash_build(0, stream_);
     synchronize();
 }
 
 uint32_t Engine::hash_table_size(int32_t s) {
-    check_seg(s);
+    Segment& sg = seg(s);
     ensure_committed();
-    return hseg_[(size_t)s].M;
+    return sg.uploaded ? sg.dev.M : 0;
 }
 
 void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint32_t* si, uint32_t* nbr) {
-    check_seg(s);
+    Segment& sg = seg(s);
     ensure_committed();
     synchronize();
-    const SegDev& d = hseg_[(size_t)s];
-    VTX_REQUIRE(d.flags & VTX_SEG_SELFCOL, VTX_ESTATE, "segment has no spatial hash");
+    VTX_REQUIRE(sg.uploaded && (sg.dev.flags & VTX_SEG_SELFCOL), VTX_ESTATE, "segment has no spatial hash");
+    const SegDev& d = sg.dev;
     if (cs || ce) {
         std::vector<uint2> tmp(d.M);
-        VTX_HIP(hipMemcpy(tmp.data(), ds_.cells + d.tbase, sizeof(uint2) * d.M, hipMemcpyDeviceToHost));
+        VTX_HIP(hipMemcpy(tmp.data(), pl_.cells + d.tbase, sizeof(uint2) * d.M, hipMemcpyDeviceToHost));
         for (uint32_t h = 0; h < d.M; h++) {
             if (cs) cs[h] = tmp[h].x;
             if (ce) ce[h] = tmp[h].y;
         }
     }
-    if (sh) VTX_HIP(hipMemcpy(sh, ds_.keys[sorted_src_] + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
-    if (si) VTX_HIP(hipMemcpy(si, ds_.vals[sorted_src_] + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
+    const uint32_t* keys = sorted_src_ ? pl_.k1 : pl_.k0;
+    const uint32_t* vals = sorted_src_ ? pl_.v1 : pl_.v0;
+    if (sh) VTX_HIP(hipMemcpy(sh, keys + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
+    if (si) VTX_HIP(hipMemcpy(si, vals + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
     if (nbr) {
         const uint64_t nb = (uint64_t)d.nbase_hi << 32 | d.nbase_lo;
-        VTX_HIP(hipMemcpy(nbr, ds_.nbr + nb, sizeof(uint32_t) * (size_t)d.max_nbr * d.n, hipMemcpyDeviceToHost));
+        VTX_HIP(hipMemcpy(nbr, pl_.nbr + nb, sizeof(uint32_t) * (size_t)d.max_nbr * d.n, hipMemcpyDeviceToHost));
     }
+}
+
+std::vector<int32_t> Engine::nonfinite_segments() {
+    ensure_committed();
+    std::vector<int32_t> bad;
+    if (table_seg_.empty()) return bad;
+    VTX_HIP(launch_check_finite(ds_, flags_, (uint32_t)table_seg_.size(), stream_));
+    std::vector<uint32_t> h(table_seg_.size());
+    VTX_HIP(hipMemcpyAsync(h.data(), flags_, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (size_t k = 0; k < h.size(); k++)
+        if (h[k]) bad.push_back(table_seg_[k]);
+    return bad;
 }
 
 }  // namespace vtx

@@ -1,8 +1,9 @@
-// Host engine behind the C ABI: registration, device layout, frame recording, graphs.
+// Host engine behind the C ABI: registration, device pools, frame recording, graphs.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -24,16 +25,53 @@ struct Error : std::runtime_error {
             throw ::vtx::Error(VTX_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-#define VTX_REQUIRE(cond, code, msg)                  \
-    do {                                              \
+#define VTX_REQUIRE(cond, code, msg)                    \
+    do {                                                \
         if (!(cond)) throw ::vtx::Error((code), (msg)); \
     } while (0)
 
+// First-fit allocator of index ranges [0, cap) with coalescing frees.
+class RangeAlloc {
+public:
+    size_t cap = 0;
+    bool take(size_t n, size_t* off);          // false if no free range is large enough
+    void give(size_t off, size_t n);
+    void grow(size_t new_cap);                 // extends the free space at the end
+    size_t used() const { return used_; }
+private:
+    std::map<size_t, size_t> free_;            // offset -> length
+    size_t used_ = 0;
+};
+
+// A segment's device data, prepared on the host with segment-local indices and offsets, so it
+// can be uploaded into any ranges of the pools without rewriting.
+struct Prepared {
+    bool grid = false;
+    uint32_t grows = 0, gcols = 0, grid_flags = 0;
+    std::vector<float4> x, v, x0;
+    std::vector<uint2> est, eat;
+    std::vector<uint32_t> ebd, etr;
+    std::vector<uint4> brec, trec;
+    std::vector<float> brest;
+    std::vector<float4> slots, rest4, brest4;
+    std::vector<uint32_t> gflags;
+    std::vector<ColliderDev> cols;
+    std::vector<TileDev> tiles;                 // ELL offsets relative to the segment's ranges
+    std::vector<GridTileDev> gtiles;
+};
+
+// Ranges a segment occupies in the device pools (offset, length).
+struct Ranges {
+    size_t p = 0, cell = 0, nbr = 0, st = 0, at = 0, bd = 0, tr = 0, brec = 0, trec = 0, slot = 0, col = 0;
+    size_t n_p = 0, n_cell = 0, n_nbr = 0, n_st = 0, n_at = 0, n_bd = 0, n_tr = 0, n_brec = 0, n_trec = 0,
+           n_slot = 0, n_col = 0;
+};
+
 // One independent simulation: the state of one reference VtClothSolverGPU
-// (Velvet/VtClothSolverGPU.hpp:207-231), kept on the host until it is committed.
+// (Velvet/VtClothSolverGPU.hpp:207-231).
 struct Segment {
     vtx_params prm{};
-    std::vector<float> x, v, w, x0;          // 3n, 3n, n, 3n
+    std::vector<float> x, v, w, x0;          // 3n, 3n, n, 3n (host copy; see `uploaded`)
     std::vector<uint32_t> tris;              // 3T, local indices
     std::vector<int32_t> st;                 // 2S
     std::vector<float> st_rest;              // S
@@ -44,10 +82,15 @@ struct Segment {
     std::vector<float> slots;                // 3L
     std::vector<vtx_collider> cols;
     std::vector<uint32_t> cloth_starts;      // first particle of each AddCloth
+    bool active = true;                      // simulated by the solver's frames
+    bool uploaded = false;                   // device ranges hold the current state
+    bool prepared = false;                   // `prep` matches the host data
+    Prepared prep;
+    Ranges rg;
+    SegDev dev{};                            // absolute bases filled at upload
     uint32_t n() const { return (uint32_t)w.size(); }
-    uint32_t table_size() const {
-        return prm.hash_table_size > 0 ? (uint32_t)prm.hash_table_size : 2u * n();
-    }
+    uint32_t table_size() const { return prm.hash_table_size > 0 ? (uint32_t)prm.hash_table_size : 2u * n(); }
+    bool selfcol() const { return prm.enable_self_collision && n() > 0; }
 };
 
 struct StageTime {
@@ -62,8 +105,10 @@ public:
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    int32_t add_segment(const vtx_params& p);
-    int32_t num_segments() const { return (int32_t)segs_.size(); }
+    int32_t add_segment(const vtx_params& p, bool active = true);
+    void remove_segment(int32_t s);
+    void set_active(int32_t s, bool active);
+    int32_t num_segments() const;
     Segment& seg(int32_t i);
 
     void set_params(int32_t s, const vtx_params& p);
@@ -76,12 +121,15 @@ public:
     void add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count);
     void set_colliders(int32_t s, const vtx_collider* cols, uint32_t n);
 
+    void prepare(int32_t s);                 // host packing ahead of upload (optional)
+    size_t device_bytes_estimate(int32_t s);
     void simulate(float frame_dt);
     void synchronize();
     void set_use_graph(bool on);
     void set_stencil_path(bool on);
     int32_t stencil_segments();
     void set_stream(hipStream_t s);
+    hipStream_t stream() const { return stream_; }
     void set_timing(bool on);
     std::string timing_report();
 
@@ -95,46 +143,65 @@ public:
     uint32_t hash_table_size(int32_t s);
     void read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint32_t* si, uint32_t* nbr);
 
-    // layout facts for measurement
-    uint32_t total_particles() const { return ds_.nparticles; }
-    uint32_t sort_passes() const { return (uint32_t)passes_; }
+    // Segments with a non-finite position or velocity after the last frame.
+    std::vector<int32_t> nonfinite_segments();
+    uint64_t live_particles() const { return live_particles_; }
+    size_t pool_bytes() const;
 
 private:
-    void pull_state();
-    void mark_dirty();
+    void pull_state(Segment& sg);
+    void touch(int32_t s);                   // a structural change of segment s
+    void invalidate_graph();
     void ensure_committed();
-    void commit();
-    void free_device();
-    void upload_colliders();
+    void upload(Segment& sg);
+    void release(Segment& sg);
+    void rebuild_tables();
+    void grow_pools(const Ranges& need);
     void record_frame(float frame_dt, hipStream_t s);
     void hash_build(int in, hipStream_t s);
     void stage_begin(const char* label, hipStream_t s);
     void stage_end(hipStream_t s);
     void check_seg(int32_t s) const;
-    template <class T>
-    T* dalloc(size_t count);
-    template <class T>
-    T* dupload(const std::vector<T>& h);
+    void free_tables();
+    void free_pools();
 
-    std::vector<Segment> segs_;
-    bool dirty_ = true;        // host layout changed since the last commit
-    bool dev_live_ = false;    // device holds the authoritative particle state
+    std::vector<std::unique_ptr<Segment>> segs_;
+    bool tables_dirty_ = true;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
 
+    // device pools: every array of a pool is indexed by that pool's allocator
+    RangeAlloc ap_, acell_, anbr_, ast_, aat_, abd_, atr_, abrec_, atrec_, aslot_, acol_;
+    struct Pools {
+        float4 *X = nullptr, *V = nullptr, *P0 = nullptr, *P1 = nullptr, *X0 = nullptr, *N = nullptr,
+               *sP = nullptr, *sX0 = nullptr, *rest4 = nullptr, *brest4 = nullptr;
+        uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *gflags = nullptr;
+        uint2* cells = nullptr;
+        uint32_t* nbr = nullptr;
+        uint2 *st = nullptr, *at = nullptr;
+        uint32_t *bd = nullptr, *tr = nullptr;
+        uint4 *brec = nullptr, *trec = nullptr;
+        float* brest = nullptr;
+        float4* slots = nullptr;
+        ColliderDev* cols = nullptr;
+    } pl_;
+    std::vector<void*> table_allocs_;
+    size_t hist_cap_ = 0, part_cap_ = 0, flag_cap_ = 0;
+    uint32_t* hist_ = nullptr;
+    uint32_t* part_ = nullptr;
+    uint32_t* flags_ = nullptr;
+
     DevState ds_{};
-    std::vector<void*> allocs_;
-    std::vector<SegDev> hseg_;
-    std::vector<TileDev> htile_;
+    std::vector<int32_t> table_seg_;         // table index -> segment id
     int passes_ = 1;
     int sorted_src_ = 0;
     bool any_selfcol_ = false;
     bool need_collide_ = false;
     int substeps_ = 1, iterations_ = 0, interleave_ = 1;
+    uint64_t live_particles_ = 0;
 
     bool use_graph_ = true;
     bool use_stencil_ = true;
-    std::vector<uint8_t> seg_grid_;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;
 

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, 
         }
         __syncthreads();
         if (valid) {
-            const uint32_t dst = wcnt[wave][dg] + rank;
+            const uint32_t dst = wcnt[wave][dg] + rank + t.bias;
             kout[dst] = key;
             vout[dst] = val;
         }
@@ -194,6 +194,17 @@ __global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, 
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+// Mark every cell of every self-colliding segment empty: lane i of a segment clears cells
+// i, i + n, i + 2n, ... < M (the reference's memset, Velvet/SpatialHashGPU.cu:185).
+__global__ __launch_bounds__(VTX_TILE) void k_clear_cells(DevState d) {
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if (threadIdx.x >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    if (!(sg.flags & VTX_SEG_SELFCOL)) return;
+    uint2* __restrict__ cells = d.cells + sg.tbase;
+    for (uint32_t i = t.begin - sg.pbase + threadIdx.x; i < sg.M; i += sg.n) cells[i] = make_uint2(VTX_INVALID, VTX_INVALID);
 }
 
 // FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) per segment with local sorted
@@ -319,6 +330,11 @@ hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s) {
     uint32_t blocks = (n + 1023) / 1024;
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(1024), 0, s, a, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_cells(const DevState& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_cells, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d);
     return hipGetLastError();
 }
 

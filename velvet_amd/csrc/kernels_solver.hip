@@ -277,25 +277,28 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
         for (uint32_t k = 0; k < t.st_k; k++) {
             const uint2 e = st[(size_t)k * VTX_TILE];
             if (e.x == VTX_INVALID) continue;
-            stretch_term(p, w, P[e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
+            stretch_term(p, w, P[sg.pbase + e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
         }
         const uint2* __restrict__ at = d.at + t.at_off + lane;
         for (uint32_t k = 0; k < t.at_k; k++) {
             const uint2 e = at[(size_t)k * VTX_TILE];
             if (e.x == VTX_INVALID) continue;
-            attach_term(p, w, d.slots[e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+            attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
         }
         const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
         const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
         for (uint32_t k = 0; k < t.bd_k; k++) {
             const uint32_t e = bd[(size_t)k * VTX_TILE];
             if (e == VTX_INVALID) continue;
-            const uint32_t bid = e & 0x3FFFFFFFu;
+            const uint32_t bid = sg.bbase + (e & 0x3FFFFFFFu);
             const uint32_t role = e >> 30;
             const uint4 ix = d.brec[bid];
+            const uint32_t pb = sg.pbase;
             float lam;
             V3 d0, d1, d2, d3;
-            if (!bend_eval(P[ix.x], P[ix.y], P[ix.z], P[ix.w], d.brest[bid], xpbd_bend, lam, d0, d1, d2, d3)) continue;
+            if (!bend_eval(P[pb + ix.x], P[pb + ix.y], P[pb + ix.z], P[pb + ix.w], d.brest[bid], xpbd_bend, lam, d0,
+                           d1, d2, d3))
+                continue;
             const V3 dk = role == 0 ? d0 : role == 1 ? d1 : role == 2 ? d2 : d3;
             delta += (-w * lam) * dk;
             cnt++;
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         for (uint32_t k = 0; k < t.at_k; k++) {
             const uint2 e = at[(size_t)k * VTX_TILE];
             if (e.x == VTX_INVALID) continue;
-            attach_term(p, w, d.slots[e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+            attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
         }
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
@@ -475,17 +478,37 @@ __global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d) {
     const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + lane;
     V3 s = v3(0.f, 0.f, 0.f);
     const uint32_t* __restrict__ tr = d.tr + t.tr_off + lane;
     for (uint32_t k = 0; k < t.tr_k; k++) {
         const uint32_t e = tr[(size_t)k * VTX_TILE];
         if (e == VTX_INVALID) continue;
-        const uint4 ix = d.trec[e];
-        const V3 a = xyz(d.X[ix.x]), b = xyz(d.X[ix.y]), c = xyz(d.X[ix.z]);
+        const uint4 ix = d.trec[sg.trbase + e];
+        const uint32_t pb = sg.pbase;
+        const V3 a = xyz(d.X[pb + ix.x]), b = xyz(d.X[pb + ix.y]), c = xyz(d.X[pb + ix.z]);
         s += cross(b - a, c - a);
     }
     d.Nrm[g] = f4(normalize(s), 0.f);
+}
+
+// Failure detection: flags[t.seg] = 1 when a particle of the segment has a non-finite position
+// or velocity (the reference has no such check; its only net is the speed clamp, cu:399-404).
+__global__ void k_clear_flags(uint32_t* flags, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = 0;
+}
+
+__global__ __launch_bounds__(VTX_TILE) void k_check_finite(DevState d, uint32_t* flags) {
+    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    bool bad = false;
+    if (threadIdx.x < t.count) {
+        const uint32_t g = t.begin + threadIdx.x;
+        const float4 x = d.X[g], v = d.V[g];
+        bad = !(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z));
+    }
+    if (__any(bad) && (threadIdx.x & 63u) == 0) flags[t.seg] = 1u;   // same value from any wave: no race on the result
 }
 
 // ---------------------------------------------------------------- host launchers
@@ -526,6 +549,12 @@ hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool
         else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, true>), gr, bl, 0, s, d, d.itiles, in, h);
         else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, true>), gr, bl, 0, s, d, d.itiles, in, h);
     }
+    return last_launch();
+}
+
+hipError_t launch_check_finite(const DevState& d, uint32_t* flags, uint32_t nseg, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_flags, dim3((nseg + 255) / 256), dim3(256), 0, s, flags, nseg);
+    if (d.ntiles) hipLaunchKernelGGL(k_check_finite, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, flags);
     return last_launch();
 }
 

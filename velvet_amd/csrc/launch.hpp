@@ -13,11 +13,13 @@ hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s);
 hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints,
                           hipStream_t s);
 hipError_t launch_normals(const DevState& d, hipStream_t s);
+hipError_t launch_check_finite(const DevState& d, uint32_t* flags, uint32_t nseg, hipStream_t s);
 
 hipError_t launch_hash_keys(const DevState& d, int in, hipStream_t s);
 hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s);
 hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
+hipError_t launch_clear_cells(const DevState& d, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
 hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s);
 

@@ -1,0 +1,129 @@
+#include "server.hpp"
+
+namespace vtx {
+
+Server::Server(Engine& e, const vtx_server_config& cfg) : e_(e), cfg_(cfg) {
+    if (cfg_.hbm_budget_bytes) {
+        limit_ = (size_t)cfg_.hbm_budget_bytes;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        VTX_HIP(hipMemGetInfo(&free_b, &total_b));
+        limit_ = free_b > cfg_.hbm_reserve_bytes ? free_b - (size_t)cfg_.hbm_reserve_bytes : 0;
+    }
+    e_.set_use_graph(cfg_.use_graph != 0);
+}
+
+Job& Server::get(int32_t id) {
+    auto it = jobs_.find(id);
+    VTX_REQUIRE(it != jobs_.end(), VTX_EINVAL, "no such job");
+    return it->second;
+}
+
+const Job& Server::job(int32_t id) const {
+    auto it = jobs_.find(id);
+    VTX_REQUIRE(it != jobs_.end(), VTX_EINVAL, "no such job");
+    return it->second;
+}
+
+// A job is a pending (inactive) segment of the server's solver; the caller registers its
+// cloths and colliders through the normal solver calls on that segment, then submits it.
+int32_t Server::new_job(const vtx_params& p, int32_t frames, int32_t* seg) {
+    VTX_REQUIRE(frames >= 1, VTX_EINVAL, "a job needs at least one frame");
+    Job j;
+    j.id = next_id_++;
+    j.seg = e_.add_segment(p, /*active=*/false);
+    j.frames = frames;
+    jobs_[j.id] = j;
+    if (seg) *seg = j.seg;
+    return j.id;
+}
+
+void Server::submit(int32_t id) {
+    Job& j = get(id);
+    VTX_REQUIRE(j.status == VTX_JOB_PENDING, VTX_ESTATE, "job was already submitted");
+    e_.prepare(j.seg);                         // host-side packing happens at submission
+    j.bytes = e_.device_bytes_estimate(j.seg);
+    j.n = e_.seg(j.seg).n();
+    if (j.bytes > limit_ || (cfg_.max_live_particles && j.n > cfg_.max_live_particles)) {
+        j.status = VTX_JOB_REJECTED;           // could never be admitted
+        e_.remove_segment(j.seg);
+        return;
+    }
+    j.status = VTX_JOB_QUEUED;
+    queue_.push_back(id);
+}
+
+bool Server::fits(const Job& j) const {
+    if (resident_ + j.bytes > limit_) return false;
+    if (cfg_.max_live_particles && live_particles_ + j.n > cfg_.max_live_particles) return false;
+    if (cfg_.max_live_jobs && live_.size() >= cfg_.max_live_jobs) return false;
+    return true;
+}
+
+void Server::step(float frame_dt, vtx_server_stats* out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    vtx_server_stats st{};
+    // admission, first come first served
+    while (!queue_.empty()) {
+        Job& j = get(queue_.front());
+        if (!fits(j)) break;
+        queue_.pop_front();
+        e_.set_active(j.seg, true);
+        j.status = VTX_JOB_RUNNING;
+        resident_ += j.bytes;
+        live_particles_ += j.n;
+        live_.push_back(j.id);
+        st.admitted++;
+    }
+    uint64_t work = 0;
+    if (!live_.empty()) {
+        e_.simulate(frame_dt);
+        e_.synchronize();
+        for (int32_t id : live_) {
+            Job& j = get(id);
+            j.done++;
+            work += (uint64_t)j.n * (uint64_t)e_.seg(j.seg).prm.num_substeps;
+        }
+    }
+    steps_++;
+    std::vector<int32_t> bad;
+    if (cfg_.check_every > 0 && steps_ % (uint64_t)cfg_.check_every == 0 && !live_.empty())
+        bad = e_.nonfinite_segments();
+    std::vector<int32_t> keep;
+    for (int32_t id : live_) {
+        Job& j = get(id);
+        bool failed = false;
+        for (int32_t s : bad) failed |= (s == j.seg);
+        if (failed || j.done >= j.frames) {
+            if (!failed) {
+                j.result.resize((size_t)3 * j.n);
+                e_.read_particles(j.seg, 0, j.result.data());
+            }
+            j.status = failed ? VTX_JOB_FAILED : VTX_JOB_DONE;
+            e_.remove_segment(j.seg);
+            j.seg = -1;
+            resident_ -= j.bytes;
+            live_particles_ -= j.n;
+            (failed ? st.failed : st.completed)++;
+        } else {
+            keep.push_back(id);
+        }
+    }
+    live_.swap(keep);
+    st.live_jobs = (uint32_t)live_.size();
+    st.queued_jobs = (uint32_t)queue_.size();
+    st.live_particles = live_particles_;
+    st.particle_substeps = work;
+    st.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out) *out = st;
+}
+
+void Server::release(int32_t id) {
+    Job& j = get(id);
+    VTX_REQUIRE(j.status != VTX_JOB_RUNNING && j.status != VTX_JOB_QUEUED, VTX_ESTATE,
+                "job is still queued or running");
+    if (j.seg >= 0) e_.remove_segment(j.seg);
+    jobs_.erase(id);
+}
+
+}  // namespace vtx

@@ -1,0 +1,48 @@
+// Continuous-batching server over the engine: jobs (independent simulations with a frame
+// horizon) queue up, are admitted while device memory and the live-particle budget allow, all
+// admitted jobs advance one frame per step in the same launches, and finished jobs leave.
+#pragma once
+#include <chrono>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace vtx {
+
+struct Job {
+    int32_t id = -1, seg = -1;
+    int32_t frames = 0, done = 0;
+    int32_t status = VTX_JOB_PENDING;
+    size_t bytes = 0;
+    uint32_t n = 0;
+    std::vector<float> result;   // final positions (xyz) once done
+};
+
+class Server {
+public:
+    Server(Engine& e, const vtx_server_config& cfg);
+    int32_t new_job(const vtx_params& p, int32_t frames, int32_t* seg);
+    void submit(int32_t job);
+    void step(float frame_dt, vtx_server_stats* out);
+    const Job& job(int32_t id) const;
+    void release(int32_t id);
+
+private:
+    Job& get(int32_t id);
+    bool fits(const Job& j) const;
+
+    Engine& e_;
+    vtx_server_config cfg_;
+    std::map<int32_t, Job> jobs_;
+    std::deque<int32_t> queue_;
+    std::vector<int32_t> live_;
+    int32_t next_id_ = 0;
+    size_t limit_ = 0;           // device bytes jobs may occupy
+    size_t resident_ = 0;        // estimated bytes of the live jobs
+    uint64_t live_particles_ = 0;
+    uint64_t steps_ = 0;
+};
+
+}  // namespace vtx

@@ -58,6 +58,7 @@ struct alignas(16) SegDev {
     float gx, gy, gz, pad0;
     uint32_t mod_lo, mod_hi;               // ceil(2^64 / M): exact remainder by multiplication
     uint32_t rows, cols;                   // grid segments: vertex grid shape
+    uint32_t bbase, trbase, pad1, pad2;    // bend-record / triangle-record bases
 };
 
 struct alignas(16) TileDev {
@@ -68,7 +69,8 @@ struct alignas(16) TileDev {
 
 struct alignas(16) SortTileDev {
     uint32_t seg, begin, count, hbase;     // hbase: histogram base of the segment
-    uint32_t ntiles, tidx, pad0, pad1;     // sort tiles of the segment, index of this one
+    uint32_t ntiles, tidx, bias, pad1;     // sort tiles of the segment, index of this one;
+                                           // bias = pbase - (particles of earlier segments), mod 2^32
 };
 
 struct alignas(16) GridTileDev {
@@ -100,13 +102,14 @@ struct DevState {
     float4* P[2];
     const float4* X0;
     float4* Nrm;
+    // ELL entries and records hold segment-local indices; kernels add the segment's bases
     const uint2* st;          // stretch ELL: (other particle, rest bits)
     const uint2* at;          // attach ELL: (slot, distance bits)
     const uint32_t* bd;       // bend ELL: bend id | role << 30
-    const uint4* brec;        // bend records: 4 global particle indices
+    const uint4* brec;        // bend records: 4 particle indices
     const float* brest;       // bend rest angles
     const uint32_t* tr;       // triangle ELL: triangle id
-    const uint4* trec;        // triangle records: 3 global particle indices
+    const uint4* trec;        // triangle records: 3 particle indices
     const float4* slots;      // attach slot positions
     const ColliderDev* cols;
     uint32_t* keys[2];        // sort keys (hash) ping-pong

@@ -21,6 +21,7 @@ same scene on every machine and numpy version.
 """
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field, replace
 from typing import List, Optional, Sequence
@@ -291,17 +292,8 @@ def grid_cloth(n: int, spacing: float, center=(0.0, 1.5, 0.0), jitter: float = 1
         x += uniform(seed, stream, n * n * 3, -jitter, jitter).reshape(-1, 3)
     x = x.astype(F32)
 
-    tris = grid_mesh(n)
-    st = _grid_stretch_fast(n)
+    tris, st, bd = grid_topology(n, bend_mode)
     rest = _length_f32(x[st[:, 0]] - x[st[:, 1]])
-    if bend_mode == "quad":
-        bd = bend_per_quad(tris)
-    elif bend_mode == "pairs":
-        bd = bend_pairs_grid(n)
-    elif bend_mode == "none":
-        bd = np.zeros((0, 4), dtype=np.uint32)
-    else:
-        raise ValueError(bend_mode)
     brest = np.zeros(bd.shape[0], dtype=F32)
 
     slots = np.zeros((len(attached), 3), dtype=F32)
@@ -323,6 +315,24 @@ def grid_cloth(n: int, spacing: float, center=(0.0, 1.5, 0.0), jitter: float = 1
     return Cloth(x=x, tris=tris, stretch=st, stretch_rest=rest, bend=bd, bend_rest=brest,
                  slots=slots, attach_pid=pid, attach_slot=sid, attach_dist=dist,
                  diameter=diam, side=n)
+
+
+@functools.lru_cache(maxsize=64)
+def grid_topology(n: int, bend_mode: str):
+    """(triangles, stretch pairs, bends) of an n x n grid; shared read-only between cloths."""
+    tris = grid_mesh(n)
+    st = _grid_stretch_fast(n)
+    if bend_mode == "quad":
+        bd = bend_per_quad(tris)
+    elif bend_mode == "pairs":
+        bd = bend_pairs_grid(n)
+    elif bend_mode == "none":
+        bd = np.zeros((0, 4), dtype=np.uint32)
+    else:
+        raise ValueError(bend_mode)
+    for a in (tris, st, bd):
+        a.setflags(write=False)
+    return tris, st, bd
 
 
 def bend_pairs_grid(n: int) -> np.ndarray:

@@ -74,9 +74,18 @@ class ClothSolver:
         if not stencil:
             self.set_stencil_path(False)
 
+    @classmethod
+    def borrow(cls, handle):
+        """Wrap a solver handle owned elsewhere (e.g. by a server); close() leaves it alone."""
+        obj = cls.__new__(cls)
+        obj.h = handle
+        obj._borrowed = True
+        return obj
+
     def close(self):
         if getattr(self, "h", None):
-            lib().vtx_solver_destroy(self.h)
+            if not getattr(self, "_borrowed", False):
+                lib().vtx_solver_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -96,6 +105,12 @@ class ClothSolver:
         out = C.c_int32()
         check(lib().vtx_add_segment(self.h, C.byref(to_vtx_params(params)), C.byref(out)))
         return out.value
+
+    def remove_segment(self, seg: int):
+        check(lib().vtx_remove_segment(self.h, seg))
+
+    def set_segment_active(self, seg: int, active: bool):
+        check(lib().vtx_set_segment_active(self.h, seg, int(bool(active))))
 
     def num_segments(self) -> int:
         out = C.c_int32()
@@ -247,21 +262,22 @@ class ClothSolver:
 
     # -- scenes
     def load_scene(self, scene: S.Scene, seg: Optional[int] = None) -> int:
-        """Register a scene into a new segment (or segment ``seg``), cloth by cloth."""
+        """Register a scene into segment ``seg``, or into the next unused segment.
+
+        Cloth by cloth, in VtClothObjectGPU::Start order: AddCloth, stretch, attach slots and
+        attachments, bending (Velvet/VtClothObjectGPU.hpp:43-58); then the colliders.
+        """
         if seg is None:
-            seg = 0 if (self.num_segments() == 1 and self.num_particles(0) == 0
-                        and not getattr(self, "_seg0_used", False)) else None
-            if seg == 0:
-                self._seg0_used = True
-                self.set_params(scene.params, 0)
+            if not getattr(self, "_seg0_used", False) and self.num_particles(0) == 0:
+                seg = 0
             else:
                 seg = self.add_segment(scene.params)
-        else:
-            self.set_params(scene.params, seg)
+        if seg == 0:
+            self._seg0_used = True
+        self.set_params(scene.params, seg)
         slot_base = 0
         for c in scene.cloths:
             off = self.AddCloth(c.x, c.tris, c.diameter, seg=seg)
-            # VtClothObjectGPU::Start order: stretch, attach, bending (ObjGPU.hpp:49-57)
             self.AddStretchN(c.stretch.astype(np.int64) + off, c.stretch_rest, seg=seg)
             for s in range(c.slots.shape[0]):
                 self.AddAttachSlot(c.slots[s], seg=seg)
@@ -270,7 +286,7 @@ class ClothSolver:
                                 c.attach_slot.astype(np.int64) + slot_base, c.attach_dist, seg=seg)
             self.AddBendN(c.bend.astype(np.int64) + off, c.bend_rest, seg=seg)
             slot_base += c.slots.shape[0]
-        # AddCloth overwrote diameter / max speed exactly as scene.params already holds them
+        # AddCloth set diameter / max speed to the values scene.params already holds
         self.set_params(scene.params, seg)
         self.UpdateColliders(scene.colliders, seg=seg)
         return seg
