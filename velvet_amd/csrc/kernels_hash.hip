@@ -17,6 +17,8 @@
 
 namespace vtx {
 
+#define VTX_NB_STAGE 16   // neighbours per lane staged in LDS before the coalesced write-out
+
 // HashCoords (Velvet/SpatialHashGPU.cu:18-22): 32-bit wrap-around products, then
 // abs(h % M). |h % M| == |h| mod M for C's truncating remainder, and the unsigned remainder is
 // taken by multiplication with c = ceil(2^64 / M): ((c * a) mod 2^64) * M >> 64 == a mod M for
@@ -244,6 +246,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
 #pragma clang fp contract(off)
     __shared__ uint32_t ranges[27][VTX_TILE];
+    __shared__ uint32_t stage[VTX_NB_STAGE][VTX_TILE];
     const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -273,7 +276,10 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int 
         mask |= (len ? 1u : 0u) << q;
         total += len;
     }
+    // accepted neighbours go to this lane's LDS column first (the first VTX_NB_STAGE of them) and
+    // are written out together at the end, one coalesced row of the column-major table at a time
     uint32_t k = 0, s = 0, left = 0;
+    bool full = false;
     for (uint32_t it = 0; it < total; it++) {
         if (left == 0) {         // next non-empty cell: one exists while it < total
             const int q = __builtin_ctz(mask);
@@ -285,21 +291,28 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int 
         const uint32_t cur = s;
         s++;
         left--;
+        // the candidate's index, position and initial position are loaded together
         const uint32_t j = sv[cur];
-        if (j == id) continue;
         const float4 qp = sP[cur];
+        const float4 r = sX0[cur];
+        if (j == id) continue;
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         if (!(d2 < sg.cell2)) continue;
-        const float4 r = sX0[cur];
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         if (!(e2 > sg.diam2)) continue;
-        nb[(size_t)k * n] = j;
+        if (k < VTX_NB_STAGE) stage[k][threadIdx.x] = j;
+        else nb[(size_t)k * n] = j;
         k++;
-        if (k >= K) return;
+        if (k >= K) {
+            full = true;
+            break;
+        }
     }
-    nb[(size_t)k * n] = VTX_INVALID;
+    const uint32_t m = k < VTX_NB_STAGE ? k : VTX_NB_STAGE;
+    for (uint32_t r = 0; r < m; r++) nb[(size_t)r * n] = stage[r][threadIdx.x];
+    if (!full) nb[(size_t)k * n] = VTX_INVALID;
 }
 
 // ---------------------------------------------------------------- host launchers

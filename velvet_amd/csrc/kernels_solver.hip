@@ -312,26 +312,28 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
 // A segment whose cloth is an R x C vertex grid with the reference's topology (structural +
 // shear stretch, bending on the X/R/D slots of the cells) runs its Jacobi pass on 16 x 16
 // tiles. The tile's positions plus a halo of 2 are staged in LDS; every bending slot touching
-// the tile is evaluated once (not once per vertex), its four corrections are kept in LDS, and
-// each vertex gathers its 8 stretch, its attachments and its 12 bending terms in a fixed order.
+// the tile is evaluated once (not once per member vertex), its four corrections are kept in
+// LDS, and each vertex gathers its 8 stretch terms, its attachments and its 12 bending terms in
+// a fixed order. The three slot types are evaluated one after another through the same LDS
+// area (X, then R, then D), which keeps the workgroup at ~21 KB of LDS.
 // Slot owner ranges relative to the tile origin (a0, b0): X rows/cols [-1, 15] (17 x 17),
 // R rows [-1, 16] cols [-1, 15] (18 x 17), D rows [-1, 15] cols [-1, 16] (17 x 18).
 #define VTX_NSX (17 * 17)
 #define VTX_NSR (18 * 17)
 #define VTX_NSD (17 * 18)
-#define VTX_NSLOT (VTX_NSX + VTX_NSR + VTX_NSD)
+#define VTX_NSMAX VTX_NSR
 
-__device__ __forceinline__ int slot_x(int ra, int rb) { return (ra + 1) * 17 + (rb + 1); }
-__device__ __forceinline__ int slot_r(int ra, int rb) { return VTX_NSX + (ra + 1) * 17 + (rb + 1); }
-__device__ __forceinline__ int slot_d(int ra, int rb) { return VTX_NSX + VTX_NSR + (ra + 1) * 18 + (rb + 1); }
+template <int TYPE>
+__device__ __forceinline__ int slot_index(int ra, int rb) {
+    return TYPE == 2 ? (ra + 1) * 18 + (rb + 1) : (ra + 1) * 17 + (rb + 1);
+}
 
 // Evaluate bending slot j (index within its type's range) of a grid tile into LDS.
 template <int TYPE>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
-                                          float (*corr)[VTX_NSLOT], uint8_t* act, int a0, int b0, int rows,
+                                          float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int rows,
                                           int cols, uint32_t j, float xpbd) {
     constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
-    constexpr int BASE = TYPE == 0 ? 0 : TYPE == 1 ? VTX_NSX : VTX_NSX + VTX_NSR;
     const int ra = (int)(j / W) - 1, rb = (int)(j % W) - 1;
     const int a = a0 + ra, b = b0 + rb;
     bool on = false;
@@ -361,16 +363,53 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
         V3 d0, d1, d2, d3;
         on = bend_eval(q0, q1, q2, q3, rest, xpbd, lam, d0, d1, d2, d3);
         if (on) {
-            const int s = BASE + (int)j;
             const V3 c0 = (-q0.w * lam) * d0, c1 = (-q1.w * lam) * d1, c2 = (-q2.w * lam) * d2,
                      c3 = (-q3.w * lam) * d3;
-            corr[0][s] = c0.x; corr[1][s] = c0.y; corr[2][s] = c0.z;
-            corr[3][s] = c1.x; corr[4][s] = c1.y; corr[5][s] = c1.z;
-            corr[6][s] = c2.x; corr[7][s] = c2.y; corr[8][s] = c2.z;
-            corr[9][s] = c3.x; corr[10][s] = c3.y; corr[11][s] = c3.z;
+            corr[0][j] = c0.x; corr[1][j] = c0.y; corr[2][j] = c0.z;
+            corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
+            corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
+            corr[9][j] = c3.x; corr[10][j] = c3.y; corr[11][j] = c3.z;
         }
     }
-    act[BASE + j] = on ? 1 : 0;
+    act[j] = on ? 1 : 0;
+}
+
+// One slot type: evaluate its slots (t, t + 256), then every tile vertex adds its four
+// memberships of this type in role order.
+template <int TYPE>
+__device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
+                                               float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
+                                               int rows, int cols, float xpbd, bool inside, int ty, int tx,
+                                               V3& delta, int& cnt) {
+    constexpr int NS = TYPE == 0 ? VTX_NSX : TYPE == 1 ? VTX_NSR : VTX_NSD;
+    const uint32_t tid = threadIdx.x;
+    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
+    if (tid < NS - VTX_TILE) eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd);
+    __syncthreads();
+    if (inside) {
+        // role r of the slot owned at (ty + dy_r, tx + dx_r), per type:
+        //   X: (0,0) (-1,-1) (0,-1) (-1,0);  R: (1,-1) (-1,0) (0,0) (0,-1);  D: (-1,1) (0,-1) (-1,0) (0,0)
+        int sidx[4];
+        if (TYPE == 0) {
+            sidx[0] = slot_index<0>(ty, tx); sidx[1] = slot_index<0>(ty - 1, tx - 1);
+            sidx[2] = slot_index<0>(ty, tx - 1); sidx[3] = slot_index<0>(ty - 1, tx);
+        } else if (TYPE == 1) {
+            sidx[0] = slot_index<1>(ty + 1, tx - 1); sidx[1] = slot_index<1>(ty - 1, tx);
+            sidx[2] = slot_index<1>(ty, tx); sidx[3] = slot_index<1>(ty, tx - 1);
+        } else {
+            sidx[0] = slot_index<2>(ty - 1, tx + 1); sidx[1] = slot_index<2>(ty, tx - 1);
+            sidx[2] = slot_index<2>(ty - 1, tx); sidx[3] = slot_index<2>(ty, tx);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int j = sidx[r];
+            if (act[j]) {
+                delta += v3(corr[3 * r][j], corr[3 * r + 1][j], corr[3 * r + 2][j]);
+                cnt++;
+            }
+        }
+    }
+    __syncthreads();   // the next type reuses corr / act
 }
 
 __device__ __forceinline__ float4 nan4() {
@@ -381,8 +420,8 @@ __device__ __forceinline__ float4 nan4() {
 template <int EPI>
 __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, float h) {
     __shared__ float4 sp[VTX_GP * VTX_GP];
-    __shared__ float corr[12][VTX_NSLOT];          // plane r*3+c: component c of role r
-    __shared__ uint8_t act[VTX_NSLOT];
+    __shared__ float corr[12][VTX_NSMAX];          // plane r*3+c: component c of role r
+    __shared__ uint8_t act[VTX_NSMAX];
     const GridTileDev t = d.gtiles[xcd_tile(blockIdx.x, gridDim.x)];
     const SegDev sg = d.segs[t.seg];
     const int rows = (int)sg.rows, cols = (int)sg.cols;
@@ -399,25 +438,10 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     }
     __syncthreads();
 
-    // evaluate every bending slot whose owner cell lies in the tile's slot ranges: thread t takes
-    // slot t of each type; the remaining 33 X, 50 R and 50 D slots go to waves 0, 1 and 2 so
-    // that every wave evaluates one slot type at a time (no divergent type dispatch)
-    const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);
-    {
-        const uint32_t wave = tid >> 6, lane = tid & 63u;
-        eval_slot<0>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
-        eval_slot<1>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
-        eval_slot<2>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd_bend);
-        if (wave == 0 && lane < VTX_NSX - VTX_TILE) eval_slot<0>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
-        if (wave == 1 && lane < VTX_NSR - VTX_TILE) eval_slot<1>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
-        if (wave == 2 && lane < VTX_NSD - VTX_TILE) eval_slot<2>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + lane, xpbd_bend);
-    }
-    __syncthreads();
-
     const int ty = (int)(tid / VTX_GT), tx = (int)(tid % VTX_GT);
     const int a = a0 + ty, b = b0 + tx;
-    if (a >= rows || b >= cols) return;
-    const uint32_t g = sg.pbase + (uint32_t)(a * cols + b);
+    const bool inside = a < rows && b < cols;
+    const uint32_t g = inside ? sg.pbase + (uint32_t)(a * cols + b) : 0u;
     const int sc = (ty + VTX_GH) * VTX_GP + (tx + VTX_GH);    // staged index of this vertex
     const float4 p4 = sp[sc];
     V3 p = xyz(p4);
@@ -425,8 +449,8 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     V3 delta = v3(0.f, 0.f, 0.f);
     int cnt = 0;
 
-    // stretch: R own/left, D own/up, X-diagonal own/up-left, anti-diagonal of left/up
-    {
+    if (inside) {
+        // stretch: R own/left, D own/up, X-diagonal own/up-left, anti-diagonal of left/up
         const float4 r_own = d.rest4[g];
         const float4 r_left = b > 0 ? d.rest4[g - 1] : nan4();
         const float4 r_up = a > 0 ? d.rest4[g - cols] : nan4();
@@ -440,34 +464,20 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
         if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
         if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
-    }
-    // attachments
-    {
+        // attachments
         const uint2* __restrict__ at = d.at + t.at_off + tid;
-        for (uint32_t k = 0; k < t.at_k; k++) {
-            const uint2 e = at[(size_t)k * VTX_TILE];
+        for (uint32_t kk = 0; kk < t.at_k; kk++) {
+            const uint2 e = at[(size_t)kk * VTX_TILE];
             if (e.x == VTX_INVALID) continue;
             attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
         }
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
-    {
-        const int sx[4] = {slot_x(ty, tx), slot_x(ty - 1, tx - 1), slot_x(ty, tx - 1), slot_x(ty - 1, tx)};
-        const int sr[4] = {slot_r(ty + 1, tx - 1), slot_r(ty - 1, tx), slot_r(ty, tx), slot_r(ty, tx - 1)};
-        const int sd[4] = {slot_d(ty - 1, tx + 1), slot_d(ty, tx - 1), slot_d(ty - 1, tx), slot_d(ty, tx)};
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (act[sx[r]]) { delta += v3(corr[3 * r][sx[r]], corr[3 * r + 1][sx[r]], corr[3 * r + 2][sx[r]]); cnt++; }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (act[sr[r]]) { delta += v3(corr[3 * r][sr[r]], corr[3 * r + 1][sr[r]], corr[3 * r + 2][sr[r]]); cnt++; }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (act[sd[r]]) { delta += v3(corr[3 * r][sd[r]], corr[3 * r + 1][sd[r]], corr[3 * r + 2][sd[r]]); cnt++; }
-        }
-    }
+    const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
+    bend_type_pass<0>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    bend_type_pass<1>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    bend_type_pass<2>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    if (!inside) return;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);
 }
