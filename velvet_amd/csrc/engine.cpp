@@ -565,7 +565,15 @@ void Engine::prepare(int32_t si) {
             pr.brest.push_back(sg.bd_rest[c]);
         }
     }
-    {
+    // dense long-range attachments: entry c = slot * n + particle for every slot and particle
+    pr.lra_dense = n > 0 && L > 0 && A == (size_t)L * n;
+    for (size_t c = 0; c < A && pr.lra_dense; c++)
+        pr.lra_dense = (size_t)sg.at_pid[c] == c % n && (size_t)sg.at_slot[c] == c / n;
+    if (pr.lra_dense) {
+        pr.lra = sg.at_dist;   // already slot-major
+        std::fill(s_at.begin(), s_at.end(), 0u);
+        l_at.clear();
+    } else {
         std::vector<uint32_t> f(s_at.begin(), s_at.end() - 1);
         for (size_t c = 0; c < A; c++)
             l_at[f[(size_t)sg.at_pid[c]]++] = make_uint2((uint32_t)sg.at_slot[c], fbits(sg.at_dist[c]));
@@ -622,13 +630,14 @@ static Ranges needs_of(const Segment& sg) {
     r.n_trec = pr.trec.size();
     r.n_slot = pr.slots.size();
     r.n_col = pr.cols.size();
+    r.n_lra = pr.lra.size();
     return r;
 }
 
 static size_t bytes_of(const Ranges& r) {
     return r.n_p * (10 * sizeof(float4) + 5 * sizeof(uint32_t)) + r.n_cell * sizeof(uint2) + r.n_nbr * 4 +
            (r.n_st + r.n_at) * sizeof(uint2) + (r.n_bd + r.n_tr) * 4 + r.n_brec * (sizeof(uint4) + 4) +
-           r.n_trec * sizeof(uint4) + r.n_slot * sizeof(float4) + r.n_col * sizeof(ColliderDev);
+           r.n_trec * sizeof(uint4) + r.n_slot * sizeof(float4) + r.n_col * sizeof(ColliderDev) + r.n_lra * 4;
 }
 
 size_t Engine::device_bytes_estimate(int32_t s) {
@@ -649,6 +658,7 @@ size_t Engine::pool_bytes() const {
     c.n_trec = atrec_.cap;
     c.n_slot = aslot_.cap;
     c.n_col = acol_.cap;
+    c.n_lra = alra_.cap;
     return bytes_of(c);
 }
 
@@ -705,12 +715,13 @@ void Engine::grow_pools(const Ranges& need) {
     grow(atrec_, need.n_trec, [&](size_t o, size_t c) { regrow(pl_.trec, o, c); });
     grow(aslot_, need.n_slot, [&](size_t o, size_t c) { regrow(pl_.slots, o, c); });
     grow(acol_, need.n_col, [&](size_t o, size_t c) { regrow(pl_.cols, o, c); });
+    grow(alra_, need.n_lra, [&](size_t o, size_t c) { regrow(pl_.lra, o, c); });
 }
 
 void Engine::free_pools() {
     void* ptrs[] = {pl_.X, pl_.V, pl_.P0, pl_.P1, pl_.X0, pl_.N, pl_.sP, pl_.sX0, pl_.rest4, pl_.brest4,
                     pl_.k0, pl_.k1, pl_.v0, pl_.v1, pl_.gflags, pl_.cells, pl_.nbr, pl_.st, pl_.at, pl_.bd,
-                    pl_.tr, pl_.brec, pl_.trec, pl_.brest, pl_.slots, pl_.cols};
+                    pl_.tr, pl_.brec, pl_.trec, pl_.brest, pl_.slots, pl_.cols, pl_.lra};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     pl_ = Pools{};
@@ -740,6 +751,8 @@ void Engine::upload(Segment& sg) {
     take(atrec_, need.n_trec, &r.trec);
     take(aslot_, need.n_slot, &r.slot);
     take(acol_, need.n_col, &r.col);
+    take(alra_, need.n_lra, &r.lra);
+    VTX_REQUIRE(r.lra + r.n_lra < (1ull << 32), VTX_ECAPACITY, "dense attachment pool exceeds 2^32 entries");
     sg.rg = r;
 
     const Prepared& pr = sg.prep;
@@ -760,7 +773,9 @@ void Engine::upload(Segment& sg) {
     d.ccount = (uint32_t)pr.cols.size();
     d.bbase = (uint32_t)r.brec;
     d.trbase = (uint32_t)r.trec;
-    d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u);
+    d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u) |
+              (pr.lra_dense ? VTX_SEG_LRA_DENSE : 0u);
+    d.lbase = (uint32_t)r.lra;
     d.max_nbr = (uint32_t)p.max_num_neighbors;
     // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
     d.diameter = p.particle_diameter;
@@ -810,6 +825,7 @@ void Engine::upload(Segment& sg) {
     h2d(pl_.trec + r.trec, pr.trec, s);
     h2d(pl_.slots + r.slot, pr.slots, s);
     h2d(pl_.cols + r.col, pr.cols, s);
+    h2d(pl_.lra + r.lra, pr.lra, s);
     VTX_HIP(hipStreamSynchronize(s));
     sg.uploaded = true;
 }
@@ -828,6 +844,7 @@ void Engine::release(Segment& sg) {
     atrec_.give(r.trec, r.n_trec);
     aslot_.give(r.slot, r.n_slot);
     acol_.give(r.col, r.n_col);
+    alra_.give(r.lra, r.n_lra);
     sg.uploaded = false;
 }
 
@@ -981,6 +998,7 @@ void Engine::rebuild_tables() {
     d.brest = pl_.brest;
     d.trec = pl_.trec;
     d.slots = pl_.slots;
+    d.lra = pl_.lra;
     d.cols = pl_.cols;
     d.hist = hist_;
     d.part = part_;
@@ -999,7 +1017,7 @@ void Engine::ensure_committed() {
         const Ranges r = needs_of(*sg);
         need.n_p += r.n_p; need.n_cell += r.n_cell; need.n_nbr += r.n_nbr; need.n_st += r.n_st;
         need.n_at += r.n_at; need.n_bd += r.n_bd; need.n_tr += r.n_tr; need.n_brec += r.n_brec;
-        need.n_trec += r.n_trec; need.n_slot += r.n_slot; need.n_col += r.n_col;
+        need.n_trec += r.n_trec; need.n_slot += r.n_slot; need.n_col += r.n_col; need.n_lra += r.n_lra;
         any = true;
     }
     if (any) {
@@ -1064,7 +1082,7 @@ void Engine::hash_build(int in, hipStream_t s) {
     VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, in, src, s));
+    VTX_HIP(launch_neighbors(ds_, s));
     stage_end(s);
     sorted_src_ = src;
 }

@@ -56,15 +56,17 @@ struct Prepared {
     std::vector<float4> slots, rest4, brest4;
     std::vector<uint32_t> gflags;
     std::vector<ColliderDev> cols;
+    std::vector<float> lra;                     // dense attachments: distances, slot-major
+    bool lra_dense = false;
     std::vector<TileDev> tiles;                 // ELL offsets relative to the segment's ranges
     std::vector<GridTileDev> gtiles;
 };
 
 // Ranges a segment occupies in the device pools (offset, length).
 struct Ranges {
-    size_t p = 0, cell = 0, nbr = 0, st = 0, at = 0, bd = 0, tr = 0, brec = 0, trec = 0, slot = 0, col = 0;
+    size_t p = 0, cell = 0, nbr = 0, st = 0, at = 0, bd = 0, tr = 0, brec = 0, trec = 0, slot = 0, col = 0, lra = 0;
     size_t n_p = 0, n_cell = 0, n_nbr = 0, n_st = 0, n_at = 0, n_bd = 0, n_tr = 0, n_brec = 0, n_trec = 0,
-           n_slot = 0, n_col = 0;
+           n_slot = 0, n_col = 0, n_lra = 0;
 };
 
 // One independent simulation: the state of one reference VtClothSolverGPU
@@ -171,7 +173,7 @@ private:
     bool own_stream_ = false;
 
     // device pools: every array of a pool is indexed by that pool's allocator
-    RangeAlloc ap_, acell_, anbr_, ast_, aat_, abd_, atr_, abrec_, atrec_, aslot_, acol_;
+    RangeAlloc ap_, acell_, anbr_, ast_, aat_, abd_, atr_, abrec_, atrec_, aslot_, acol_, alra_;
     struct Pools {
         float4 *X = nullptr, *V = nullptr, *P0 = nullptr, *P1 = nullptr, *X0 = nullptr, *N = nullptr,
                *sP = nullptr, *sX0 = nullptr, *rest4 = nullptr, *brest4 = nullptr;
@@ -184,6 +186,7 @@ private:
         float* brest = nullptr;
         float4* slots = nullptr;
         ColliderDev* cols = nullptr;
+        float* lra = nullptr;
     } pl_;
     std::vector<void*> table_allocs_;
     size_t hist_cap_ = 0, part_cap_ = 0, flag_cap_ = 0;

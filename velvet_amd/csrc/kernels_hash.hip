@@ -17,7 +17,6 @@
 
 namespace vtx {
 
-#define VTX_NB_STAGE 16   // neighbours per lane staged in LDS before the coalesced write-out
 
 // HashCoords (Velvet/SpatialHashGPU.cu:18-22): 32-bit wrap-around products, then
 // abs(h % M). |h % M| == |h| mod M for C's truncating remainder, and the unsigned remainder is
@@ -232,87 +231,73 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
         }
     }
     if (local == sg.n - 1) cells[key].y = sg.n;
+    // sorted copies for the neighbour search; w of the position copy carries the particle's
+    // local index (the inverse mass is not needed there)
     const uint32_t id = d.vals[src][g];
-    d.sP[g] = d.P[in][sg.pbase + id];
+    const float4 q = d.P[in][sg.pbase + id];
+    d.sP[g] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
     d.sX0[g] = d.X0[sg.pbase + id];
 }
 
-// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130), one thread per particle in index
-// order: neighbouring lanes hold neighbouring cloth vertices, so they probe the same cells and
-// the column-major table writes coalesce. The 27 cell entries are loaded up front (independent
-// loads), candidates come from the sorted copies (contiguous per cell). Cells are visited x-,
-// y-, z-nested; inside a cell in sorted order, at most max_nbr entries per cell and in total;
-// the list ends with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
-__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, int in, int src) {
+// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130). Cells are visited x-, y-, z-nested;
+// inside a cell in sorted order, at most max_nbr entries per cell and in total; the list ends
+// with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
+// Threads take particles in sorted order, so the lanes of one cell walk identical candidate
+// ranges and their loads merge; each list still goes to its particle's column of the
+// particle-order table. A first sweep over the 27 cells finds the non-empty ones (bit q of
+// mask) and the total candidate count; the walk is one flat loop over their concatenation in
+// the same order, reloading a cell's range when it gets there (an L1 hit).
+__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d) {
 #pragma clang fp contract(off)
-    __shared__ uint32_t ranges[27][VTX_TILE];
-    __shared__ uint32_t stage[VTX_NB_STAGE][VTX_TILE];
     const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
-    const uint32_t g = t.begin + threadIdx.x;
-    const uint32_t id = g - sg.pbase;
+    const uint32_t i = t.begin + threadIdx.x - sg.pbase;     // sorted position
     const uint32_t n = sg.n;
-    const uint32_t* __restrict__ sv = d.vals[src] + sg.pbase;
     const float4* __restrict__ sP = d.sP + sg.pbase;
     const float4* __restrict__ sX0 = d.sX0 + sg.pbase;
     const uint2* __restrict__ cells = d.cells + sg.tbase;
+    const float4 p = sP[i];
+    const float4 o = sX0[i];
+    const uint32_t id = __float_as_uint(p.w);
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    const float4 p = d.P[in][g];
-    const float4 o = d.X0[g];
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
     const uint64_t mc = seg_mod(sg);
-    // The 27 cell ranges, x-, y-, z-nested, packed as start << 7 | length (length <= 64,
-    // start < 2^25) in this lane's LDS column; the candidate walk is one flat loop over their
-    // concatenation (same order), so the lanes of a wave diverge only in the total count.
     const uint32_t K = sg.max_nbr;
     uint32_t mask = 0, total = 0;
 #pragma unroll
     for (int q = 0; q < 27; q++) {
         const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
         const uint32_t len = c.x == VTX_INVALID ? 0u : min(c.y, c.x + K) - c.x;
-        ranges[q][threadIdx.x] = c.x << 7 | len;
         mask |= (len ? 1u : 0u) << q;
         total += len;
     }
-    // accepted neighbours go to this lane's LDS column first (the first VTX_NB_STAGE of them) and
-    // are written out together at the end, one coalesced row of the column-major table at a time
     uint32_t k = 0, s = 0, left = 0;
-    bool full = false;
     for (uint32_t it = 0; it < total; it++) {
         if (left == 0) {         // next non-empty cell: one exists while it < total
             const int q = __builtin_ctz(mask);
             mask &= mask - 1;
-            const uint32_t r = ranges[q][threadIdx.x];
-            s = r >> 7;
-            left = r & 127u;
+            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
+            s = c.x;
+            left = min(c.y, c.x + K) - c.x;
         }
         const uint32_t cur = s;
         s++;
         left--;
-        // the candidate's index, position and initial position are loaded together
-        const uint32_t j = sv[cur];
+        if (cur == i) continue;                              // the particle itself
         const float4 qp = sP[cur];
-        const float4 r = sX0[cur];
-        if (j == id) continue;
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         if (!(d2 < sg.cell2)) continue;
+        const float4 r = sX0[cur];
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         if (!(e2 > sg.diam2)) continue;
-        if (k < VTX_NB_STAGE) stage[k][threadIdx.x] = j;
-        else nb[(size_t)k * n] = j;
-        k++;
-        if (k >= K) {
-            full = true;
-            break;
-        }
+        nb[(size_t)k * n] = __float_as_uint(qp.w);
+        if (++k >= K) return;                                // full: no sentinel
     }
-    const uint32_t m = k < VTX_NB_STAGE ? k : VTX_NB_STAGE;
-    for (uint32_t r = 0; r < m; r++) nb[(size_t)r * n] = stage[r][threadIdx.x];
-    if (!full) nb[(size_t)k * n] = VTX_INVALID;
+    nb[(size_t)k * n] = VTX_INVALID;
 }
 
 // ---------------------------------------------------------------- host launchers
@@ -356,8 +341,8 @@ hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
+hipError_t launch_neighbors(const DevState& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d);
     return hipGetLastError();
 }
 

@@ -196,6 +196,38 @@ __device__ __forceinline__ void attach_term(V3 p, float w, float4 slot, float di
     }
 }
 
+#define VTX_LRA_BATCH 16
+
+// All attachments of one particle, in registration order: dense (slot-major distances, the slot
+// position is the same for every lane) or from the tile's attach ELL.
+__device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg, uint32_t at_off, uint32_t at_k,
+                                             uint32_t lane, uint32_t local, V3 p, float w, V3& delta, int& cnt) {
+    if (sg.flags & VTX_SEG_LRA_DENSE) {
+        // Distances are fetched VTX_LRA_BATCH slots at a time so that many loads are in flight
+        // per lane (a large cloth may give only one wave per SIMD); the terms are still summed
+        // in slot order.
+        const float* __restrict__ lra = d.lra + sg.lbase + local;
+        const float4* __restrict__ slot = d.slots + sg.sbase;
+        const size_t n = sg.n;
+        uint32_t s = 0;
+        for (; s + VTX_LRA_BATCH <= sg.scount; s += VTX_LRA_BATCH) {
+            float r[VTX_LRA_BATCH];
+#pragma unroll
+            for (int u = 0; u < VTX_LRA_BATCH; u++) r[u] = lra[(size_t)(s + u) * n];
+#pragma unroll
+            for (int u = 0; u < VTX_LRA_BATCH; u++) attach_term(p, w, slot[s + u], r[u], sg.lrs, delta, cnt);
+        }
+        for (; s < sg.scount; s++) attach_term(p, w, slot[s], lra[(size_t)s * n], sg.lrs, delta, cnt);
+        return;
+    }
+    const uint2* __restrict__ at = d.at + at_off + lane;
+    for (uint32_t k = 0; k < at_k; k++) {
+        const uint2 e = at[(size_t)k * VTX_TILE];
+        if (e.x == VTX_INVALID) continue;
+        attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+    }
+}
+
 // The dihedral bending constraint (Velvet/VtClothSolverGPU.cu:128-177). With c1, c2 the raw
 // triangle normals, the reference's n1 = c1 / |c1|^2 and normalize(n1) = c1 / |c1| are formed
 // from one reciprocal square root each. Returns false on the reference's early-outs; else
@@ -279,12 +311,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             if (e.x == VTX_INVALID) continue;
             stretch_term(p, w, P[sg.pbase + e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
         }
-        const uint2* __restrict__ at = d.at + t.at_off + lane;
-        for (uint32_t k = 0; k < t.at_k; k++) {
-            const uint2 e = at[(size_t)k * VTX_TILE];
-            if (e.x == VTX_INVALID) continue;
-            attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
-        }
+        attach_terms(d, sg, t.at_off, t.at_k, lane, g - sg.pbase, p, w, delta, cnt);
         const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
         const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
         for (uint32_t k = 0; k < t.bd_k; k++) {
@@ -464,13 +491,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
         if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
         if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
-        // attachments
-        const uint2* __restrict__ at = d.at + t.at_off + tid;
-        for (uint32_t kk = 0; kk < t.at_k; kk++) {
-            const uint2 e = at[(size_t)kk * VTX_TILE];
-            if (e.x == VTX_INVALID) continue;
-            attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
-        }
+        attach_terms(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)

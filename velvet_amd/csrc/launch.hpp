@@ -21,6 +21,6 @@ hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_clear_cells(const DevState& d, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
-hipError_t launch_neighbors(const DevState& d, int in, int src, hipStream_t s);
+hipError_t launch_neighbors(const DevState& d, hipStream_t s);
 
 }  // namespace vtx

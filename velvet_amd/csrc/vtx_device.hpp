@@ -32,6 +32,9 @@
 #define VTX_SEG_NONE_X 64u
 #define VTX_SEG_NONE_R 128u
 #define VTX_SEG_NONE_D 256u
+// Attachments are "every particle to every slot, in slot order" (Velvet/VtClothObjectGPU.hpp:134-148):
+// only the distances are stored, slot-major: lra[lbase + slot * n + particle].
+#define VTX_SEG_LRA_DENSE 512u
 
 // 2-D grid tiles of the stencil path: VTX_GT x VTX_GT particles, halo of 2 rows/cols.
 #define VTX_GT 16
@@ -58,7 +61,7 @@ struct alignas(16) SegDev {
     float gx, gy, gz, pad0;
     uint32_t mod_lo, mod_hi;               // ceil(2^64 / M): exact remainder by multiplication
     uint32_t rows, cols;                   // grid segments: vertex grid shape
-    uint32_t bbase, trbase, pad1, pad2;    // bend-record / triangle-record bases
+    uint32_t bbase, trbase, lbase, pad2;   // bend-record / triangle-record / dense-LRA bases
 };
 
 struct alignas(16) TileDev {
@@ -111,13 +114,14 @@ struct DevState {
     const uint32_t* tr;       // triangle ELL: triangle id
     const uint4* trec;        // triangle records: 3 particle indices
     const float4* slots;      // attach slot positions
+    const float* lra;         // dense long-range attachment distances
     const ColliderDev* cols;
     uint32_t* keys[2];        // sort keys (hash) ping-pong
     uint32_t* vals[2];        // sort values (local particle index) ping-pong
     uint32_t* hist;           // per (segment, digit, sort tile) counts, scanned in place
     uint32_t* part;           // scan partials
     uint2* cells;             // per hash cell: (start, end) in sorted order, start ~0u = empty
-    float4* sP;               // predicted positions gathered into sorted order
+    float4* sP;               // predicted positions in sorted order, w = local particle index
     float4* sX0;              // initial positions gathered into sorted order
     uint32_t* nbr;            // neighbour tables, column-major per segment
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
