@@ -152,7 +152,7 @@ def cpu_baseline(scenes, budget_s):
     return done_ps / spent, spent, frames, cloths
 
 
-def serve_run(rate, seconds, seed, max_live_particles, use_graph, ahead=4):
+def serve_run(rate, seconds, seed, max_live_particles, use_graph, step_budget_ms=0.0, ahead=4):
     """Poisson arrivals at `rate` req/s for `seconds` of wall time against the serving loop.
 
     Arrived requests wait in a backlog (descriptors only); the driver materialises and submits
@@ -166,7 +166,8 @@ def serve_run(rate, seconds, seed, max_live_particles, use_graph, ahead=4):
     arrived = submitted = admitted = completed = 0
     k = 0
     next_t = rng.exponential(1.0 / rate)
-    with Server(max_live_particles=max_live_particles, use_graph=use_graph) as sv:
+    with Server(max_live_particles=max_live_particles, use_graph=use_graph,
+                step_budget_ms=step_budget_ms) as sv:
         queued = 0
         t0 = time.perf_counter()
         while True:
@@ -198,7 +199,7 @@ def serve_run(rate, seconds, seed, max_live_particles, use_graph, ahead=4):
             "particle_substeps_per_s": work / wall, "arrived": arrived, "admitted": admitted,
             "completed": completed, "backlog": len(backlog) + queued,
             "mean_live_particles": float(np.mean(live)) if live else 0.0,
-            "max_live_particles_cap": max_live_particles}
+            "max_live_particles_cap": max_live_particles, "step_budget_ms": step_budget_ms}
 
 
 def main():
@@ -215,7 +216,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-rates", default="50", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
-    ap.add_argument("--serve-cap", type=int, default=3_000_000, help="live-particle cap (latency budget)")
+    ap.add_argument("--serve-cap", type=int, default=0, help="live-particle cap; 0 = none")
+    ap.add_argument("--serve-budget-ms", type=float, default=11.0,
+                    help="admission budget on the predicted frame time")
     ap.add_argument("--serve-graph", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -274,7 +277,8 @@ def main():
         serving = []
         for r in [float(x) for x in args.serve_rates.split(",") if x]:
             log(f"serving: {r:g} req/s for {args.serve_seconds:g} s, cap {args.serve_cap} live particles")
-            serving.append(serve_run(r, args.serve_seconds, 0, args.serve_cap, bool(args.serve_graph)))
+            serving.append(serve_run(r, args.serve_seconds, 0, args.serve_cap, bool(args.serve_graph),
+                                     args.serve_budget_ms))
             log(f"  {serving[-1]}")
 
     cpu = None

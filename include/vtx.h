@@ -207,7 +207,8 @@ typedef struct vtx_server_config {
     uint32_t max_live_jobs;        /* 0 = none */
     int32_t use_graph;             /* replay frames as hipGraphs (re-recorded on membership change) */
     int32_t check_every;           /* non-finite check period in steps; 0 = never */
-    int32_t pad;
+    float step_budget_ms;          /* admit while the predicted frame time stays within it (cost
+                                      per particle-substep measured on the live steps); 0 = none */
 } vtx_server_config;
 
 typedef struct vtx_server_stats {

@@ -93,6 +93,35 @@ def test_admission_budget_and_queue():
         assert sv.status(js_)[0] == DONE
 
 
+def test_step_time_budget_admission():
+    """A budget below any frame's cost admits one job per empty server: jobs run one at a time
+    and each result still equals its solo run; a roomy budget runs them together."""
+    say("serving step-time budget")
+    js = jobs(4)
+    with Server(step_budget_ms=1e-6) as sv:
+        ids = [sv.submit(sc, f) for sc, f in js]
+        most = 0
+        for _ in range(200):
+            st = sv.step()
+            most = max(most, st.live_jobs)
+            if all(sv.status(j)[0] == DONE for j in ids):
+                break
+        assert most == 1
+        for (sc, f), j in zip(js, ids):
+            assert sv.status(j)[0] == DONE
+            assert np.array_equal(sv.result(j), alone(sc, f))
+    with Server(step_budget_ms=1e4) as sv:
+        ids = [sv.submit(sc, f) for sc, f in js]
+        most = 0
+        for _ in range(200):
+            st = sv.step()
+            most = max(most, st.live_jobs)
+            if all(sv.status(j)[0] == DONE for j in ids):
+                break
+        assert most == len(js)      # one job on the first step (no measurement yet), then the rest
+        assert all(sv.status(j)[0] == DONE for j in ids)
+
+
 def test_nonfinite_job_is_evicted():
     say("serving failure detection")
     js = jobs(3)

@@ -35,7 +35,7 @@ class VtxCollider(C.Structure):
 class VtxServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("hbm_reserve_bytes", C.c_uint64),
                 ("max_live_particles", C.c_uint64), ("max_live_jobs", u32), ("use_graph", i32),
-                ("check_every", i32), ("pad", i32)]
+                ("check_every", i32), ("step_budget_ms", f32)]
 
 
 class VtxServerStats(C.Structure):

@@ -44,6 +44,7 @@ void Server::submit(int32_t id) {
     e_.prepare(j.seg);                         // host-side packing happens at submission
     j.bytes = e_.device_bytes_estimate(j.seg);
     j.n = e_.seg(j.seg).n();
+    j.work = (uint64_t)j.n * (uint64_t)e_.seg(j.seg).prm.num_substeps;
     if (j.bytes > limit_ || (cfg_.max_live_particles && j.n > cfg_.max_live_particles)) {
         j.status = VTX_JOB_REJECTED;           // could never be admitted
         e_.remove_segment(j.seg);
@@ -53,10 +54,17 @@ void Server::submit(int32_t id) {
     queue_.push_back(id);
 }
 
-bool Server::fits(const Job& j) const {
+// Step-time admission: the frame time of the live set is predicted as (particle-substeps) x
+// (measured ms per particle-substep). Until a step has been measured, one job is admitted per
+// step. A job always fits an empty server (so one that alone exceeds the budget still runs).
+bool Server::fits(const Job& j, uint32_t admitted_now) const {
     if (resident_ + j.bytes > limit_) return false;
     if (cfg_.max_live_particles && live_particles_ + j.n > cfg_.max_live_particles) return false;
     if (cfg_.max_live_jobs && live_.size() >= cfg_.max_live_jobs) return false;
+    if (cfg_.step_budget_ms > 0.f && !live_.empty()) {
+        if (ms_per_work_ == 0.0) return admitted_now == 0;
+        if ((double)(live_work_ + j.work) * ms_per_work_ > (double)cfg_.step_budget_ms) return false;
+    }
     return true;
 }
 
@@ -66,23 +74,30 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
     // admission, first come first served
     while (!queue_.empty()) {
         Job& j = get(queue_.front());
-        if (!fits(j)) break;
+        if (!fits(j, st.admitted)) break;
         queue_.pop_front();
         e_.set_active(j.seg, true);
         j.status = VTX_JOB_RUNNING;
         resident_ += j.bytes;
         live_particles_ += j.n;
+        live_work_ += j.work;
         live_.push_back(j.id);
         st.admitted++;
     }
     uint64_t work = 0;
     if (!live_.empty()) {
+        const auto f0 = std::chrono::steady_clock::now();
         e_.simulate(frame_dt);
         e_.synchronize();
+        const double frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
         for (int32_t id : live_) {
             Job& j = get(id);
             j.done++;
-            work += (uint64_t)j.n * (uint64_t)e_.seg(j.seg).prm.num_substeps;
+            work += j.work;
+        }
+        if (work) {
+            const double r = frame_ms / (double)work;
+            ms_per_work_ = ms_per_work_ == 0.0 ? r : 0.8 * ms_per_work_ + 0.2 * r;
         }
     }
     steps_++;
@@ -104,6 +119,7 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
             j.seg = -1;
             resident_ -= j.bytes;
             live_particles_ -= j.n;
+            live_work_ -= j.work;
             (failed ? st.failed : st.completed)++;
         } else {
             keep.push_back(id);

@@ -17,6 +17,7 @@ struct Job {
     int32_t status = VTX_JOB_PENDING;
     size_t bytes = 0;
     uint32_t n = 0;
+    uint64_t work = 0;           // particle-substeps per frame
     std::vector<float> result;   // final positions (xyz) once done
 };
 
@@ -31,7 +32,7 @@ public:
 
 private:
     Job& get(int32_t id);
-    bool fits(const Job& j) const;
+    bool fits(const Job& j, uint32_t admitted_now) const;
 
     Engine& e_;
     vtx_server_config cfg_;
@@ -42,6 +43,8 @@ private:
     size_t limit_ = 0;           // device bytes jobs may occupy
     size_t resident_ = 0;        // estimated bytes of the live jobs
     uint64_t live_particles_ = 0;
+    uint64_t live_work_ = 0;     // particle-substeps per frame of the live jobs
+    double ms_per_work_ = 0.0;   // measured frame time per particle-substep (EMA); 0 = not yet
     uint64_t steps_ = 0;
 };
 
