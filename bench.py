@@ -4,8 +4,9 @@
 Default workload (``--workload batch512``): 512 independent cloths, sides log-uniform 16..128
 (seeded), each with its own stiffness, spacing, 0-4 attached corners and 1-3 sphere colliders,
 5 substeps x 10 iterations, self-collision on. Batched = every cloth a segment of one solver,
-one launch per stage, each frame replayed as a hipGraph. Plain = one solver per cloth, each
-launching its own kernels per stage on one shared stream, no graph.
+one launch per stage; before warm-up the launch-variant tuner picks graph replay, neighbour
+search order, attachment load batch and XCD remap (all bit-identical). Plain = one solver per
+cloth, each launching its own kernels per stage on one shared stream, no graph.
 
 A step is one 1/60 s frame of every cloth. ``value`` = sum(particles) x substeps x steps /
 seconds of the timed region (inputs resident in HBM). The line also carries the plain path's
@@ -212,6 +213,8 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=30)
     ap.add_argument("--kernel-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--tune-frames", type=int, default=3,
+                    help="frames per variant for the launch-variant tuner before warm-up; 0 = defaults")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-rates", default="50", help="comma-separated Poisson rates (req/s)")
@@ -238,6 +241,11 @@ def main():
     solver = ClothSolver(use_graph=True)
     for sc in scenes:
         solver.load_scene(sc)
+    tuning = solver.tuning()
+    if args.tune_frames > 0:
+        log(f"tuning launch variants ({args.tune_frames} frames each)")
+        tuning = solver.autotune(frames=args.tune_frames)
+        log(f"  {tuning}")
     log("registered; warming up")
     for _ in range(args.warmup):
         solver.Simulate()
@@ -297,7 +305,7 @@ def main():
             "data": "synthetic (seeded cloths)",
             "config": {"workload": f"{args.workload}: {desc}", "particles": tot["N"],
                        "stretch": tot["S"], "bend": tot["B"], "attach": tot["A"],
-                       "substeps": substeps, "iterations": iters, "graph": True},
+                       "substeps": substeps, "iterations": iters, "tuning": tuning},
             "plain": plain,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "frames": len(lat)},

@@ -142,6 +142,26 @@ int vtx_set_use_graph(vtx_solver* s, int32_t use_graph);
  * lists. vtx_stencil_segments reports how many segments took the stencil layout. */
 int vtx_set_stencil_path(vtx_solver* s, int32_t enable);
 int vtx_stencil_segments(vtx_solver* s, int32_t* count);
+/* Extension (no reference counterpart; the reference has fixed launch shapes,
+ * Velvet/Common.cuh:23-31): launch variants of one frame. Every combination gives bit-identical
+ * results; only speed differs. */
+typedef struct vtx_tuning {
+    int32_t use_graph;         /* replay frames from a hipGraph */
+    int32_t neighbor_order;    /* neighbour search thread order: 0 particle index, 1 hash-sorted,
+                                  2 vertex patches for grid cloths (sorted for other meshes) */
+    int32_t lra_batch;         /* dense attachment distances loaded per batch: 4 or 16 */
+    int32_t xcd_remap;         /* hand each XCD a contiguous run of tiles */
+    float frame_ms;            /* vtx_autotune: the chosen variant's frame time (0 from the cache) */
+    int32_t pad;
+} vtx_tuning;
+int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);
+int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);
+/* Time the variants on the current live set (coordinate descent, `frames` timed frames each,
+ * positions and velocities restored after every trial, so the simulation does not advance),
+ * apply the fastest and cache it for the live set's shape class (log2 buckets of particles and
+ * segments, hash / stencil / dense-attachment presence, substeps x iterations). use_cache = 1
+ * applies a class tuned earlier in this process without timing. */
+int vtx_autotune(vtx_solver* s, int32_t frames, float frame_dt, int32_t use_cache, vtx_tuning* out);
 /* Bind the solver to an existing stream (hipStream_t), or NULL for the solver's own. */
 int vtx_set_stream(vtx_solver* s, void* hip_stream);
 

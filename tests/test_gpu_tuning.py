@@ -1,0 +1,102 @@
+"""Launch variants and the tuner on the GPU through the C ABI.
+
+* every combination of the tuning knobs (graph replay, neighbour-search thread order, dense
+  attachment batch, XCD remap) gives bit-identical positions and velocities, on a batch that has
+  grid cloths (patch order) and on one whose segments are general meshes (two cloths sharing a
+  segment);
+* vtx_autotune leaves the simulation state where it was (trials are undone) and a later run
+  under the tuned variant equals an untuned run bit for bit;
+* a second solver of the same shape class takes the cached tuning without timing.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from velvet_amd import scenes as S
+from velvet_amd.solver import ClothSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _batch():
+    # self-collision, 0-4 dense attachment slots, 1-3 spheres per cloth
+    return S.config4(count=6, frames=4, smax=48)
+
+
+def _general():
+    # two cloths in one segment: a shared hash and the general (incidence-list) path
+    sc = S.config1(frames=4)
+    c2 = S.grid_cloth(12, 0.05, center=(0.0, 1.55, 0.0), seed=5, stream=9)
+    return [S.Scene(sc.params, [sc.cloths[0], c2], sc.colliders, 4)]
+
+
+def _run(scenes, frames, **tuning):
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        if tuning:
+            s.set_tuning(**tuning)
+        for _ in range(frames):
+            s.Simulate()
+        s.synchronize()
+        return [(s.positions(k), s.velocities(k)) for k in range(len(scenes))]
+
+
+@pytest.mark.parametrize("which", ["grid batch", "general"])
+def test_variants_bit_identical(which):
+    scenes = _batch() if which == "grid batch" else _general()
+    ref = _run(scenes, 4)
+    for g, so, lb, rm in itertools.product((0, 1), (0, 1, 2), (4, 16), (0, 1)):
+        say(f"{which}: graph={g} neighbor_order={so} lra_batch={lb} remap={rm}")
+        got = _run(scenes, 4, use_graph=g, neighbor_order=so, lra_batch=lb, xcd_remap=rm)
+        for k, ((x, v), (xr, vr)) in enumerate(zip(got, ref)):
+            assert np.array_equal(x, xr) and np.array_equal(v, vr), (k, g, so, lb, rm)
+
+
+def test_variants_bit_identical_long_range_attachments():
+    say("variants on config 2 (256 dense slots)")
+    sc = S.config2(frames=2)
+    ref = _run([sc], 2)
+    got = _run([sc], 2, lra_batch=4, neighbor_order=0, xcd_remap=0)
+    assert np.array_equal(got[0][0], ref[0][0]) and np.array_equal(got[0][1], ref[0][1])
+
+
+def test_autotune_keeps_state_and_caches():
+    say("autotune")
+    scenes = _batch()
+    ref = _run(scenes, 3)
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        x0 = [s.positions(k) for k in range(len(scenes))]
+        t = s.autotune(frames=2)
+        say(f"  tuned {t}")
+        assert t["frame_ms"] > 0 and t["lra_batch"] in (4, 16)
+        for k in range(len(scenes)):
+            assert np.array_equal(s.positions(k), x0[k])
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        for k in range(len(scenes)):
+            assert np.array_equal(s.positions(k), ref[k][0]) and np.array_equal(s.velocities(k), ref[k][1])
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        c = s.autotune(frames=2, use_cache=True)
+        assert c["frame_ms"] == 0.0
+        assert {k: c[k] for k in ClothSolver._TUNING_KEYS} == {k: t[k] for k in ClothSolver._TUNING_KEYS}
+
+
+def test_tuning_rejects_bad_values():
+    with ClothSolver() as s:
+        with pytest.raises(Exception):
+            s.set_tuning(lra_batch=7)
+        with pytest.raises(Exception):
+            s.set_tuning(neighbor_order=3)

@@ -38,6 +38,11 @@ class VtxServerConfig(C.Structure):
                 ("check_every", i32), ("step_budget_ms", f32)]
 
 
+class VtxTuning(C.Structure):
+    _fields_ = [("use_graph", i32), ("neighbor_order", i32), ("lra_batch", i32), ("xcd_remap", i32),
+                ("frame_ms", f32), ("pad", i32)]
+
+
 class VtxServerStats(C.Structure):
     _fields_ = [("live_jobs", u32), ("queued_jobs", u32), ("admitted", u32), ("completed", u32),
                 ("failed", u32), ("pad", u32), ("live_particles", C.c_uint64),
@@ -69,6 +74,9 @@ _SIGS = {
     "vtx_simulate": (i32, [C.c_void_p, f32]),
     "vtx_synchronize": (i32, [C.c_void_p]),
     "vtx_set_use_graph": (i32, [C.c_void_p, i32]),
+    "vtx_get_tuning": (i32, [C.c_void_p, P(VtxTuning)]),
+    "vtx_set_tuning": (i32, [C.c_void_p, P(VtxTuning)]),
+    "vtx_autotune": (i32, [C.c_void_p, i32, f32, i32, P(VtxTuning)]),
     "vtx_set_stream": (i32, [C.c_void_p, C.c_void_p]),
     "vtx_set_stencil_path": (i32, [C.c_void_p, i32]),
     "vtx_stencil_segments": (i32, [C.c_void_p, P(i32)]),

@@ -204,6 +204,41 @@ int vtx_set_use_graph(vtx_solver* s, int32_t use_graph) {
     API_BEGIN s->engine.set_use_graph(use_graph != 0); API_END
 }
 
+static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
+    out->use_graph = t.graph ? 1 : 0;
+    out->neighbor_order = t.nbr_order;
+    out->lra_batch = t.lra_batch;
+    out->xcd_remap = t.remap ? 1 : 0;
+    out->frame_ms = (float)ms;
+    out->pad = 0;
+}
+
+int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
+    NEED(s); NEED(out);
+    API_BEGIN to_abi(s->engine.tuning(), 0.0, out); API_END
+}
+
+int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
+    NEED(s); NEED(t);
+    API_BEGIN
+    vtx::Tuning k;
+    k.graph = t->use_graph != 0;
+    k.nbr_order = t->neighbor_order;
+    k.lra_batch = t->lra_batch;
+    k.remap = t->xcd_remap != 0;
+    s->engine.set_tuning(k);
+    API_END
+}
+
+int vtx_autotune(vtx_solver* s, int32_t frames, float frame_dt, int32_t use_cache, vtx_tuning* out) {
+    NEED(s);
+    API_BEGIN
+    double ms = 0.0;
+    const vtx::Tuning t = s->engine.autotune(frames, frame_dt, use_cache != 0, &ms);
+    if (out) to_abi(t, ms, out);
+    API_END
+}
+
 int vtx_set_stencil_path(vtx_solver* s, int32_t enable) {
     NEED(s);
     API_BEGIN s->engine.set_stencil_path(enable != 0); API_END

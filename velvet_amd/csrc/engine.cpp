@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <sstream>
 
 #include "launch.hpp"
@@ -884,6 +885,7 @@ void Engine::rebuild_tables() {
     std::vector<SortTileDev> hst;
     table_seg_.clear();
     any_selfcol_ = false;
+    any_lra_dense_ = false;
     need_collide_ = false;
     live_particles_ = 0;
     uint32_t maxbits = 0, hbase = 0, compact = 0;
@@ -911,6 +913,7 @@ void Engine::rebuild_tables() {
             maxbits = std::max(maxbits, std::max(1u, ceil_log2(d.M)));
         }
         if ((d.flags & VTX_SEG_SELFCOL) || d.ccount) need_collide_ = true;
+        if (d.flags & VTX_SEG_LRA_DENSE) any_lra_dense_ = true;
         for (TileDev t : sg.prep.tiles) {
             t.seg = ti;
             t.begin += d.pbase;
@@ -999,6 +1002,7 @@ void Engine::rebuild_tables() {
     d.trec = pl_.trec;
     d.slots = pl_.slots;
     d.lra = pl_.lra;
+    d.remap = tun_.remap ? 1u : 0u;
     d.cols = pl_.cols;
     d.hist = hist_;
     d.part = part_;
@@ -1082,7 +1086,7 @@ void Engine::hash_build(int in, hipStream_t s) {
     VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, s));
+    VTX_HIP(launch_neighbors(ds_, in, tun_.nbr_order, s));
     stage_end(s);
     sorted_src_ = src;
 }
@@ -1107,14 +1111,14 @@ void Engine::record_frame(float frame_dt, hipStream_t s) {
         const int epi_last = last_sub ? EPILOGUE_FINALIZE : EPILOGUE_FINALIZE_PREDICT;
         if (iterations_ == 0) {
             stage_begin("k_iterate", s);
-            VTX_HIP(launch_iterate(ds_, cur, h, epi_last, false, s));
+            VTX_HIP(launch_iterate(ds_, cur, h, epi_last, false, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
         for (int it = 0; it < iterations_; it++) {
             const int epi = it == iterations_ - 1 ? epi_last : EPILOGUE_NONE;
             stage_begin("k_iterate", s);
-            VTX_HIP(launch_iterate(ds_, cur, h, epi, true, s));
+            VTX_HIP(launch_iterate(ds_, cur, h, epi, true, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
@@ -1140,7 +1144,7 @@ void Engine::simulate(float frame_dt) {
             timing_ms_["total"] += ms;
             timing_n_["total"] += 1;
         }
-    } else if (use_graph_) {
+    } else if (tun_.graph) {
         if (!gexec_ || graph_dt_ != frame_dt) {
             invalidate_graph();
             hipGraph_t graph = nullptr;
@@ -1167,8 +1171,133 @@ void Engine::simulate(float frame_dt) {
 void Engine::synchronize() { VTX_HIP(hipStreamSynchronize(stream_)); }
 
 void Engine::set_use_graph(bool on) {
-    use_graph_ = on;
-    if (!on) invalidate_graph();
+    tun_.graph = on;
+    invalidate_graph();
+}
+
+void Engine::set_tuning(const Tuning& t) {
+    VTX_REQUIRE(t.lra_batch == 4 || t.lra_batch == 16, VTX_EINVAL, "lra_batch must be 4 or 16");
+    VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
+    tun_ = t;
+    ds_.remap = t.remap ? 1u : 0u;
+    invalidate_graph();
+}
+
+// Shape class of the live set: log2 buckets of particles and segments, whether the hash,
+// the stencil path and dense attachments are present, and the substep / iteration counts.
+std::string Engine::shape_class() {
+    ensure_committed();
+    uint32_t nseg = 0, ngrid = 0;
+    for (auto& sg : segs_)
+        if (sg && sg->active && sg->uploaded && sg->n()) {
+            nseg++;
+            if (sg->prep.grid) ngrid++;
+        }
+    std::ostringstream o;
+    const uint32_t np = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, live_particles_), UINT32_MAX);
+    o << "p" << ceil_log2(np) << ".s" << ceil_log2(std::max(1u, nseg))
+      << ".h" << (any_selfcol_ ? 1 : 0) << ".g" << (ngrid ? 1 : 0) << ".l" << (any_lra_dense_ ? 1 : 0) << "."
+      << substeps_ << "x" << iterations_;
+    return o.str();
+}
+
+namespace {
+struct TunedClass {
+    Tuning t;
+    double ms;
+};
+std::mutex g_tuned_mu;
+std::map<std::string, TunedClass> g_tuned;   // shape class -> tuning, shared by the process's solvers
+}  // namespace
+
+Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* frame_ms) {
+    VTX_REQUIRE(frames >= 1 && frame_dt > 0.f, VTX_EINVAL, "autotune needs frames >= 1 and frame_dt > 0");
+    ensure_committed();
+    if (frame_ms) *frame_ms = 0.0;
+    if (ds_.nparticles == 0) return tun_;
+    const std::string key = shape_class();
+    if (use_cache) {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            set_tuning(it->second.t);
+            return tun_;
+        }
+    }
+    const size_t n = ap_.cap;
+    float4 *sx = nullptr, *sv = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timing = timing_;
+    timing_ = false;
+    auto cleanup = [&] {
+        (void)hipStreamSynchronize(stream_);
+        if (sx) (void)hipFree(sx);
+        if (sv) (void)hipFree(sv);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        timing_ = timing;
+    };
+    Tuning best = tun_;
+    double bms = 0.0;
+    try {
+        VTX_HIP(hipMalloc((void**)&sx, n * sizeof(float4)));
+        VTX_HIP(hipMalloc((void**)&sv, n * sizeof(float4)));
+        VTX_HIP(hipEventCreate(&e0));
+        VTX_HIP(hipEventCreate(&e1));
+        VTX_HIP(hipMemcpyAsync(sx, pl_.X, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        VTX_HIP(hipMemcpyAsync(sv, pl_.V, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        auto restore = [&] {
+            VTX_HIP(hipMemcpyAsync(pl_.X, sx, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+            VTX_HIP(hipMemcpyAsync(pl_.V, sv, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        };
+        auto trial = [&](const Tuning& t) {
+            set_tuning(t);
+            simulate(frame_dt);                 // warm-up, records the graph
+            restore();
+            VTX_HIP(hipEventRecord(e0, stream_));
+            for (int f = 0; f < frames; f++) simulate(frame_dt);
+            VTX_HIP(hipEventRecord(e1, stream_));
+            restore();
+            VTX_HIP(hipStreamSynchronize(stream_));
+            float ms = 0.f;
+            VTX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / frames;
+        };
+        bms = trial(best);
+        // one knob at a time, every other value of it against the best so far
+        for (int knob = 0; knob < 4; knob++) {
+            if (knob == 1 && !any_selfcol_) continue;
+            if (knob == 2 && !any_lra_dense_) continue;
+            const int nval = knob == 1 ? 3 : 2;
+            const Tuning base = best;
+            for (int v = 0; v < nval; v++) {
+                Tuning t = base;
+                if (knob == 0) t.graph = v != 0;
+                if (knob == 1) t.nbr_order = v;
+                if (knob == 2) t.lra_batch = v ? 16 : 4;
+                if (knob == 3) t.remap = v != 0;
+                if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
+                                : knob == 2 ? t.lra_batch == base.lra_batch : t.remap == base.remap)
+                    continue;
+                const double ms = trial(t);
+                if (ms < 0.99 * bms) {
+                    best = t;
+                    bms = ms;
+                }
+            }
+        }
+        set_tuning(best);
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        g_tuned[key] = TunedClass{best, bms};
+    }
+    if (frame_ms) *frame_ms = bms;
+    return tun_;
 }
 
 void Engine::set_stream(hipStream_t s) {

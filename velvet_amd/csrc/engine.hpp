@@ -95,6 +95,16 @@ struct Segment {
     bool selfcol() const { return prm.enable_self_collision && n() > 0; }
 };
 
+// Launch variants the tuner chooses among. Every combination gives bit-identical results: they
+// change which workgroup takes which tile, the order threads take particles in, how many loads
+// are in flight and whether launches are replayed from a graph, never the arithmetic.
+struct Tuning {
+    bool graph = true;        // replay frames from a hipGraph
+    int nbr_order = 2;        // neighbour search thread order: 0 particle, 1 sorted, 2 grid patches
+    int lra_batch = 16;       // dense attachment distances loaded per batch: 4 or 16
+    bool remap = true;        // XCD-aware workgroup -> tile remap
+};
+
 struct StageTime {
     std::string label;
     hipEvent_t a, b;
@@ -128,6 +138,15 @@ public:
     void simulate(float frame_dt);
     void synchronize();
     void set_use_graph(bool on);
+    const Tuning& tuning() const { return tun_; }
+    void set_tuning(const Tuning& t);
+    // Times the variants (coordinate descent, `frames` timed frames each, the state restored
+    // after every trial so the simulation does not advance), applies the fastest and caches it
+    // for the current shape class; with use_cache a class tuned before in this process is applied
+    // without timing. Returns the applied tuning; *frame_ms = its frame time (0 from the cache
+    // when it was not measured here).
+    Tuning autotune(int frames, float frame_dt, bool use_cache, double* frame_ms);
+    std::string shape_class();
     void set_stencil_path(bool on);
     int32_t stencil_segments();
     void set_stream(hipStream_t s);
@@ -203,7 +222,8 @@ private:
     int substeps_ = 1, iterations_ = 0, interleave_ = 1;
     uint64_t live_particles_ = 0;
 
-    bool use_graph_ = true;
+    Tuning tun_;
+    bool any_lra_dense_ = false;
     bool use_stencil_ = true;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;

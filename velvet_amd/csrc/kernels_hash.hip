@@ -40,7 +40,7 @@ __device__ __forceinline__ int32_t int_coord(float v, float cell) {
 
 // ComputeParticleHash_Kernel (Velvet/SpatialHashGPU.cu:34-42) over predicted positions.
 __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -200,7 +200,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
 // Mark every cell of every self-colliding segment empty: lane i of a segment clears cells
 // i, i + n, i + 2n, ... < M (the reference's memset, Velvet/SpatialHashGPU.cu:185).
 __global__ __launch_bounds__(VTX_TILE) void k_clear_cells(DevState d) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_clear_cells(DevState d) {
 // positions (cells were filled with ~0u: empty), plus the gather of predicted and initial
 // positions into sorted order for the neighbour search.
 __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, int src) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
@@ -242,25 +242,42 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130). Cells are visited x-, y-, z-nested;
 // inside a cell in sorted order, at most max_nbr entries per cell and in total; the list ends
 // with ~0u unless full. Squared distances are fmaf(dz,dz,fmaf(dy,dy,dx*dx)).
-// Threads take particles in sorted order, so the lanes of one cell walk identical candidate
-// ranges and their loads merge; each list still goes to its particle's column of the
-// particle-order table. A first sweep over the 27 cells finds the non-empty ones (bit q of
+// Thread -> particle mapping (ORDER), all giving the same table:
+//   NBR_PARTICLE: index order over the segment's particle tiles;
+//   NBR_SORTED:   hash-sorted order, so the lanes of one cell walk identical candidate ranges and
+//                 their loads merge;
+//   NBR_PATCH:    16 x 16 vertex patches of a grid cloth (its stencil tiles): a wave is 4 rows x
+//                 16 columns, so lanes share cells and candidates, and each row of the
+//                 column-major table is written as four 64-byte runs. A first sweep over the 27 cells finds the non-empty ones (bit q of
 // mask) and the total candidate count; the walk is one flat loop over their concatenation in
 // the same order, reloading a cell's range when it gets there (an L1 hit).
-__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d) {
+template <int ORDER>
+__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
 #pragma clang fp contract(off)
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    if (threadIdx.x >= t.count) return;
-    const SegDev sg = d.segs[t.seg];
+    uint32_t seg, i;
+    if (ORDER == NBR_PATCH) {
+        const GridTileDev t = d.gtiles[tile_of(d.remap)];
+        seg = t.seg;
+        const SegDev& s0 = d.segs[seg];
+        const uint32_t a = t.a0 + threadIdx.x / VTX_GT, b = t.b0 + threadIdx.x % VTX_GT;
+        if (a >= s0.rows || b >= s0.cols) return;
+        i = a * s0.cols + b;
+    } else {
+        const TileDev t = tiles[tile_of(d.remap)];
+        if (threadIdx.x >= t.count) return;
+        seg = t.seg;
+        i = t.begin + threadIdx.x - d.segs[seg].pbase;         // sorted position or particle
+    }
+    const SegDev sg = d.segs[seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
-    const uint32_t i = t.begin + threadIdx.x - sg.pbase;     // sorted position
     const uint32_t n = sg.n;
     const float4* __restrict__ sP = d.sP + sg.pbase;
     const float4* __restrict__ sX0 = d.sX0 + sg.pbase;
     const uint2* __restrict__ cells = d.cells + sg.tbase;
-    const float4 p = sP[i];
-    const float4 o = sX0[i];
-    const uint32_t id = __float_as_uint(p.w);
+    const bool sorted = ORDER == NBR_SORTED;
+    const float4 p = sorted ? sP[i] : d.P[in][sg.pbase + i];
+    const float4 o = sorted ? sX0[i] : d.X0[sg.pbase + i];
+    const uint32_t id = sorted ? __float_as_uint(p.w) : i;
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
     const uint64_t mc = seg_mod(sg);
@@ -285,8 +302,9 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d) {
         const uint32_t cur = s;
         s++;
         left--;
-        if (cur == i) continue;                              // the particle itself
         const float4 qp = sP[cur];
+        const uint32_t j = __float_as_uint(qp.w);
+        if (j == id) continue;                               // the particle itself
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         if (!(d2 < sg.cell2)) continue;
@@ -294,7 +312,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d) {
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         if (!(e2 > sg.diam2)) continue;
-        nb[(size_t)k * n] = __float_as_uint(qp.w);
+        nb[(size_t)k * n] = j;
         if (++k >= K) return;                                // full: no sentinel
     }
     nb[(size_t)k * n] = VTX_INVALID;
@@ -341,8 +359,16 @@ hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const DevState& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_neighbors, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d);
+hipError_t launch_neighbors(const DevState& d, int in, int order, hipStream_t s) {
+    const dim3 bl(VTX_TILE);
+    if (order == NBR_PARTICLE) {
+        hipLaunchKernelGGL(k_neighbors<NBR_PARTICLE>, dim3(d.ntiles), bl, 0, s, d, d.tiles, in);
+    } else if (order == NBR_SORTED) {
+        hipLaunchKernelGGL(k_neighbors<NBR_SORTED>, dim3(d.ntiles), bl, 0, s, d, d.tiles, in);
+    } else {   // grid segments by patches, the others in sorted order
+        if (d.ngtiles) hipLaunchKernelGGL(k_neighbors<NBR_PATCH>, dim3(d.ngtiles), bl, 0, s, d, d.itiles, in);
+        if (d.nitiles) hipLaunchKernelGGL(k_neighbors<NBR_SORTED>, dim3(d.nitiles), bl, 0, s, d, d.itiles, in);
+    }
     return hipGetLastError();
 }
 

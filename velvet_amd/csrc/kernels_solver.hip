@@ -16,16 +16,25 @@
 
 namespace vtx {
 
-// ComputeFriction (Velvet/VtClothSolverGPU.cu:272-287).
+// Hardware reciprocal / reciprocal square root / square root (1 ulp) for the solver passes.
+// (The library is built with correctly rounded division and square root, which the hash needs
+// for bit-exact cell coordinates; the solver passes are held to the FP64 oracle's tolerance.)
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float flen(V3 a) { return fsqrt(dot(a, a)); }
+
+// ComputeFriction (Velvet/VtClothSolverGPU.cu:272-287). tl == 0 gives mt * inf = inf, clamped
+// to 1 as the reference's mt / 0 is.
 __device__ __forceinline__ V3 friction_of(V3 correction, V3 rel, float friction) {
     V3 f = v3(0.f, 0.f, 0.f);
-    float cl = length(correction);
+    float cl = flen(correction);
     if (friction > 0.f && cl > 0.f) {
-        V3 nrm = correction / cl;
+        V3 nrm = correction * frcp(cl);
         V3 tan = rel - nrm * dot(rel, nrm);
-        float tl = length(tan);
+        float tl = flen(tan);
         float mt = cl * friction;
-        f = -tan * fminf(mt / tl, 1.0f);
+        f = -tan * fminf(mt * frcp(tl), 1.0f);
     }
     return f;
 }
@@ -40,16 +49,17 @@ __device__ V3 compute_sdf(const ColliderDev& c, V3 p, float margin) {
     } else if (c.type == 0) {
         float radius = c.scale[0] + margin;
         V3 diff = p - v3(c.pos[0], c.pos[1], c.pos[2]);
-        float distance = length(diff);
+        float distance = flen(diff);
         float offset = distance - radius;
         if (offset < 0.f) {
-            V3 direction = diff / distance;
+            V3 direction = diff * frcp(distance);
             return -offset * direction;
         }
     } else if (c.type == 2) {
         V3 correction = v3(0.f, 0.f, 0.f);
         V3 local = xform_point(c.inv, p);
-        V3 size = v3(0.5f + margin / c.scale[0], 0.5f + margin / c.scale[1], 0.5f + margin / c.scale[2]);
+        V3 size = v3(0.5f + margin * frcp(c.scale[0]), 0.5f + margin * frcp(c.scale[1]),
+                     0.5f + margin * frcp(c.scale[2]));
         V3 offset = vabs(local) - size;
         float maxv = fmaxf(offset.x, fmaxf(offset.y, offset.z));
         float minv = fminf(offset.x, fminf(offset.y, offset.z));
@@ -62,9 +72,9 @@ __device__ V3 compute_sdf(const ColliderDev& c, V3 p, float margin) {
                 V3 mask = v3(offset.x < 0.f ? sgnf(local.x) : 0.f, offset.y < 0.f ? sgnf(local.y) : 0.f,
                              offset.z < 0.f ? sgnf(local.z) : 0.f);
                 V3 vec = offset + v3(m, m, m);
-                float l = length(vec);
+                float l = flen(vec);
                 if (l < m) {
-                    V3 nv = normalize(vec);
+                    V3 nv = vec * frsq(dot(vec, vec));   // glm::normalize
                     correction = v3(mask.x * nv.x * (m - l), mask.y * nv.y * (m - l), mask.z * nv.z * (m - l));
                 }
             } else if (offset.x == maxv) {
@@ -91,7 +101,7 @@ __device__ V3 collide_sdf(V3 pred, V3 pos, const ColliderDev* cols, uint32_t cb,
         pred += corr;
         if (dot(corr, corr) > 0.f) {
             V3 lastp = xform_point(col.rel, pred);
-            V3 vel = (pred - lastp) / col.dt;
+            V3 vel = (pred - lastp) * frcp(col.dt);
             V3 relv = pred - pos - vel * dt;
             pred += friction_of(corr, relv, friction);
         }
@@ -102,7 +112,7 @@ __device__ V3 collide_sdf(V3 pred, V3 pos, const ColliderDev* cols, uint32_t cb,
 // Pre-stabilisation SDF on positions with the frame time (Velvet/VtClothSolverGPU.hpp:73) and
 // the first substep's PredictPositions (cu:42-53).
 __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float frame_dt, float h) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -120,7 +130,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float fram
 // CollideParticles_Kernel + ApplyDeltas (Velvet/VtClothSolverGPU.cu:329-386) followed by the
 // substep's CollideSDF (cu:289-314, hpp:87).
 __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float h) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
@@ -145,17 +155,17 @@ __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float 
             if (denom <= 0.f) continue;
             const V3 pj = xyz(pj4);
             const V3 diff = p - pj;
-            const float dist = length(diff);
+            const float dist = flen(diff);
             if (dist >= sg.diameter) continue;
-            const V3 grad = diff / (dist + VTX_EPSILON);
-            const float lambda = (dist - sg.diameter) / denom;
+            const V3 grad = diff * frcp(dist + VTX_EPSILON);
+            const float lambda = (dist - sg.diameter) * frcp(denom);
             const V3 common = lambda * grad;
             cnt++;
             delta -= w * common;
             const V3 relv = vel_i - (pj - xyz(d.X[jg]));
             delta += w * friction_of(common, relv, sg.friction);
         }
-        if (cnt > 0) p += delta / (float)cnt * sg.omega;
+        if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     }
     if (sg.ccount) p = collide_sdf(p, x, d.cols, sg.cbase, sg.ccount, sg.margin, sg.friction, h);
     d.P[in ^ 1][g] = f4(p, w);
@@ -163,11 +173,6 @@ __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float 
 
 // Epilogue modes of the Jacobi pass.
 enum { EPI_NONE = 0, EPI_FINALIZE = 1, EPI_FINALIZE_PREDICT = 2 };
-
-// Hardware reciprocal / reciprocal square root / square root (1 ulp) for the Jacobi pass.
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // One stretch constraint seen from one endpoint (Velvet/VtClothSolverGPU.cu:80-98): bilateral,
 // compliance 0, eps in the gradient denominator. q = other endpoint (xyz, inverse mass).
@@ -196,26 +201,25 @@ __device__ __forceinline__ void attach_term(V3 p, float w, float4 slot, float di
     }
 }
 
-#define VTX_LRA_BATCH 16
-
 // All attachments of one particle, in registration order: dense (slot-major distances, the slot
 // position is the same for every lane) or from the tile's attach ELL.
+template <int LB>
 __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg, uint32_t at_off, uint32_t at_k,
                                              uint32_t lane, uint32_t local, V3 p, float w, V3& delta, int& cnt) {
     if (sg.flags & VTX_SEG_LRA_DENSE) {
-        // Distances are fetched VTX_LRA_BATCH slots at a time so that many loads are in flight
-        // per lane (a large cloth may give only one wave per SIMD); the terms are still summed
-        // in slot order.
+        // Distances are fetched LB slots at a time so that several loads are in flight per lane
+        // (a large cloth may give only one wave per SIMD); the terms are still summed in slot
+        // order, so every LB gives the same result.
         const float* __restrict__ lra = d.lra + sg.lbase + local;
         const float4* __restrict__ slot = d.slots + sg.sbase;
         const size_t n = sg.n;
         uint32_t s = 0;
-        for (; s + VTX_LRA_BATCH <= sg.scount; s += VTX_LRA_BATCH) {
-            float r[VTX_LRA_BATCH];
+        for (; s + LB <= sg.scount; s += LB) {
+            float r[LB];
 #pragma unroll
-            for (int u = 0; u < VTX_LRA_BATCH; u++) r[u] = lra[(size_t)(s + u) * n];
+            for (int u = 0; u < LB; u++) r[u] = lra[(size_t)(s + u) * n];
 #pragma unroll
-            for (int u = 0; u < VTX_LRA_BATCH; u++) attach_term(p, w, slot[s + u], r[u], sg.lrs, delta, cnt);
+            for (int u = 0; u < LB; u++) attach_term(p, w, slot[s + u], r[u], sg.lrs, delta, cnt);
         }
         for (; s < sg.scount; s++) attach_term(p, w, slot[s], lra[(size_t)s * n], sg.lrs, delta, cnt);
         return;
@@ -228,35 +232,41 @@ __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg
     }
 }
 
-// The dihedral bending constraint (Velvet/VtClothSolverGPU.cu:128-177). With c1, c2 the raw
-// triangle normals, the reference's n1 = c1 / |c1|^2 and normalize(n1) = c1 / |c1| are formed
-// from one reciprocal square root each. Returns false on the reference's early-outs; else
-// lam carries the sign rule and the correction of vertex k is -w_k * lam * d_k.
+// The dihedral bending constraint (Velvet/VtClothSolverGPU.cu:128-177), evaluated through the
+// edge vector e = p3 - p2, a = p2 - p0 and b = p3 - p1:
+//   c1 = (p2-p0) x (p3-p0) = a x e,   c2 = (p3-p1) x (p2-p1) = e x b,
+//   (p0-p3).e = -(a.e) - |e|^2,        (p1-p3).e = -(b.e),
+//   (c1 x c2).e = |e|^2 (a.c2)         (since e.c2 = 0), so the sign rule needs one dot product.
+// The reference's n1 = c1 / |c1|^2 and normalize(n1) = c1 / |c1| come from one reciprocal square
+// root each. Returns false on the reference's early-outs; else lam carries the sign rule and the
+// correction of vertex k is -w_k * lam * d_k.
 __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float4 q3, float rest, float xpbd,
                                           float& lam, V3& d0, V3& d1, V3& d2, V3& d3) {
     const V3 p0 = xyz(q0), p1 = xyz(q1), p2 = xyz(q2), p3 = xyz(q3);
     const V3 e = p3 - p2;
-    const float elen = fsqrt(dot(e, e));
+    const float ee = dot(e, e);
+    const float elen = fsqrt(ee);
     if (elen < VTX_EPSILON) return false;
     const float inv_elen = frcp(elen);
-    const V3 c1 = cross(p2 - p0, p3 - p0);
-    const V3 c2 = cross(p3 - p1, p2 - p1);
+    const V3 a = p2 - p0, b = p3 - p1;
+    const V3 c1 = cross(a, e);
+    const V3 c2 = cross(e, b);
     const float r1 = frsq(dot(c1, c1)), r2 = frsq(dot(c2, c2));
     const V3 u1 = c1 * r1, u2 = c2 * r2;
     const V3 n1 = u1 * r1, n2 = u2 * r2;
     d0 = elen * n1;
     d1 = elen * n2;
-    d2 = (dot(p0 - p3, e) * inv_elen) * n1 + (dot(p1 - p3, e) * inv_elen) * n2;
+    d2 = ((-dot(a, e) - ee) * inv_elen) * n1 + (-dot(b, e) * inv_elen) * n2;
     // the four gradients of a dihedral angle sum to zero (it is translation invariant):
     // the reference's d3 = (p2-p0).e/|e| n1 + (p2-p1).e/|e| n2 equals -(d0 + d1 + d2)
     d3 = -(d0 + d1 + d2);
     const float cd = fminf(fmaxf(dot(u1, u2), -1.0f), 1.0f);
     const float phi = acosf(cd);
     // |d0|^2 = |e|^2 / |c1|^2 and |d1|^2 = |e|^2 / |c2|^2
-    float l = dot(e, e) * (q0.w * r1 * r1 + q1.w * r2 * r2) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
+    float l = ee * (q0.w * r1 * r1 + q1.w * r2 * r2) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
     if (l < VTX_EPSILON) return false;
     l = (phi - rest) * frcp(l + xpbd);
-    if (dot(cross(u1, u2), e) > 0.0f) l = -l;
+    if (dot(a, c2) > 0.0f) l = -l;      // (n1 x n2).e > 0
     lam = l;
     return true;
 }
@@ -270,10 +280,10 @@ __device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, ui
     } else {
         const float4 x4 = d.X[g];
         const V3 x = xyz(x4);
-        V3 raw = (p - x) / h;
-        const float rl = length(raw);
+        V3 raw = (p - x) * frcp(h);
+        const float rl = flen(raw);
         if (rl > sg.max_speed) {
-            raw = raw / rl * sg.max_speed;
+            raw = raw * (frcp(rl) * sg.max_speed);
             p = x + raw * h;
         }
         V3 v = raw * (1.0f - sg.damping * h);
@@ -289,10 +299,10 @@ __device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, ui
 // One Jacobi pass on the general path: SolveStretch (cu:65-102) + SolveAttachment
 // (cu:205-235) + SolveBending (cu:117-189) gathered per particle over its incident
 // constraints (sliced ELL), then ApplyDeltas (cu:253-264) and the epilogue.
-template <int EPI, bool CONSTRAINTS>
+template <int EPI, bool CONSTRAINTS, int LB>
 __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev* __restrict__ tiles, int in,
                                                       float h) {
-    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[tile_of(d.remap)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -311,7 +321,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             if (e.x == VTX_INVALID) continue;
             stretch_term(p, w, P[sg.pbase + e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
         }
-        attach_terms(d, sg, t.at_off, t.at_k, lane, g - sg.pbase, p, w, delta, cnt);
+        attach_terms<LB>(d, sg, t.at_off, t.at_k, lane, g - sg.pbase, p, w, delta, cnt);
         const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
         const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
         for (uint32_t k = 0; k < t.bd_k; k++) {
@@ -444,12 +454,12 @@ __device__ __forceinline__ float4 nan4() {
     return make_float4(q, q, q, q);
 }
 
-template <int EPI>
+template <int EPI, int LB>
 __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, float h) {
     __shared__ float4 sp[VTX_GP * VTX_GP];
     __shared__ float corr[12][VTX_NSMAX];          // plane r*3+c: component c of role r
     __shared__ uint8_t act[VTX_NSMAX];
-    const GridTileDev t = d.gtiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const GridTileDev t = d.gtiles[tile_of(d.remap)];
     const SegDev sg = d.segs[t.seg];
     const int rows = (int)sg.rows, cols = (int)sg.cols;
     const int a0 = (int)t.a0, b0 = (int)t.b0;
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
         if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
         if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
-        attach_terms(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
+        attach_terms<LB>(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
@@ -506,7 +516,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
 // ComputeTriangleNormals + ComputeVertexNormals (Velvet/VtClothSolverGPU.cu:419-450) as a
 // per-vertex gather over incident triangles.
 __global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -532,7 +542,7 @@ __global__ void k_clear_flags(uint32_t* flags, uint32_t n) {
 }
 
 __global__ __launch_bounds__(VTX_TILE) void k_check_finite(DevState d, uint32_t* flags) {
-    const TileDev t = d.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = d.tiles[tile_of(d.remap)];
     bool bad = false;
     if (threadIdx.x < t.count) {
         const uint32_t g = t.begin + threadIdx.x;
@@ -557,29 +567,35 @@ hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s) {
     return last_launch();
 }
 
-hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints,
-                          hipStream_t s) {
+template <int LB>
+static void launch_iterate_lb(const DevState& d, int in, float h, int epilogue, hipStream_t s) {
     dim3 bl(VTX_TILE);
-    if (!constraints) {   // no Jacobi pass: only the epilogue, over every particle
-        if (!d.ntiles) return hipSuccess;
-        dim3 gr(d.ntiles);
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, false>), gr, bl, 0, s, d, d.tiles, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, false>), gr, bl, 0, s, d, d.tiles, in, h);
-        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, false>), gr, bl, 0, s, d, d.tiles, in, h);
-        return last_launch();
-    }
     if (d.ngtiles) {
         dim3 gg(d.ngtiles);
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate_grid<EPI_NONE>), gg, bl, 0, s, d, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE>), gg, bl, 0, s, d, in, h);
-        else hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE_PREDICT>), gg, bl, 0, s, d, in, h);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate_grid<EPI_NONE, LB>), gg, bl, 0, s, d, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE, LB>), gg, bl, 0, s, d, in, h);
+        else hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE_PREDICT, LB>), gg, bl, 0, s, d, in, h);
     }
     if (d.nitiles) {
         dim3 gr(d.nitiles);
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, true>), gr, bl, 0, s, d, d.itiles, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, true>), gr, bl, 0, s, d, d.itiles, in, h);
-        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, true>), gr, bl, 0, s, d, d.itiles, in, h);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, true, LB>), gr, bl, 0, s, d, d.itiles, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, true, LB>), gr, bl, 0, s, d, d.itiles, in, h);
+        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, true, LB>), gr, bl, 0, s, d, d.itiles, in, h);
     }
+}
+
+hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints, int lra_batch,
+                          hipStream_t s) {
+    if (!constraints) {   // no Jacobi pass: only the epilogue, over every particle
+        if (!d.ntiles) return hipSuccess;
+        dim3 gr(d.ntiles), bl(VTX_TILE);
+        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, false, 1>), gr, bl, 0, s, d, d.tiles, in, h);
+        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate<EPI_FINALIZE, false, 1>), gr, bl, 0, s, d, d.tiles, in, h);
+        else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, false, 1>), gr, bl, 0, s, d, d.tiles, in, h);
+        return last_launch();
+    }
+    if (lra_batch >= 16) launch_iterate_lb<16>(d, in, h, epilogue, s);
+    else launch_iterate_lb<4>(d, in, h, epilogue, s);
     return last_launch();
 }
 

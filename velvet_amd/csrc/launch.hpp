@@ -10,7 +10,8 @@ enum { EPILOGUE_NONE = 0, EPILOGUE_FINALIZE = 1, EPILOGUE_FINALIZE_PREDICT = 2 }
 
 hipError_t launch_frame_begin(const DevState& d, float frame_dt, float h, hipStream_t s);
 hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s);
-hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints,
+// lra_batch: dense attachment distances loaded per batch (4 or 16; same results)
+hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints, int lra_batch,
                           hipStream_t s);
 hipError_t launch_normals(const DevState& d, hipStream_t s);
 hipError_t launch_check_finite(const DevState& d, uint32_t* flags, uint32_t nseg, hipStream_t s);
@@ -21,6 +22,8 @@ hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_clear_cells(const DevState& d, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
-hipError_t launch_neighbors(const DevState& d, hipStream_t s);
+// Thread -> particle order of the neighbour search (same tables for every order).
+enum { NBR_PARTICLE = 0, NBR_SORTED = 1, NBR_PATCH = 2 };
+hipError_t launch_neighbors(const DevState& d, int in, int order, hipStream_t s);
 
 }  // namespace vtx

@@ -50,4 +50,9 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// The tile the current workgroup of a tile-indexed launch processes (remap = 0: its own index).
+__device__ __forceinline__ uint32_t tile_of(uint32_t remap) {
+    return remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
 }  // namespace vtx

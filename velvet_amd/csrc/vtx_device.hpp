@@ -130,7 +130,7 @@ struct DevState {
     const uint32_t* gflags;   // grid: bending slots present at a cell (VTX_BX | VTX_BR | VTX_BD)
     const float4* brest4;     // grid: bending rest angles of a cell (X, R, D)
     uint32_t ntiles, nstiles, nhist, ncells;
-    uint32_t nparticles, nitiles, ngtiles, pad0;
+    uint32_t nparticles, nitiles, ngtiles, remap;   // remap: workgroups -> tiles through xcd_tile
 };
 
 }  // namespace vtx

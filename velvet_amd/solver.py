@@ -14,7 +14,7 @@ from typing import Iterable, List, Optional, Sequence
 import numpy as np
 
 from . import scenes as S
-from ._lib import VtxCollider, VtxParams, check, lib
+from ._lib import VtxCollider, VtxParams, VtxTuning, check, lib
 
 F32 = np.float32
 
@@ -182,6 +182,31 @@ class ClothSolver:
 
     def set_use_graph(self, on: bool):
         check(lib().vtx_set_use_graph(self.h, int(bool(on))))
+
+    # Launch variants (all give bit-identical results) and the tuner that picks among them.
+    _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap")
+
+    def tuning(self) -> dict:
+        t = VtxTuning()
+        check(lib().vtx_get_tuning(self.h, C.byref(t)))
+        return {k: int(getattr(t, k)) for k in self._TUNING_KEYS}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown tuning keys {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in kw.items()})
+        t = VtxTuning(**cur, frame_ms=0.0, pad=0)
+        check(lib().vtx_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, frames: int = 3, frame_dt: Optional[float] = None, use_cache: bool = False) -> dict:
+        t = VtxTuning()
+        check(lib().vtx_autotune(self.h, int(frames), float(S.FRAME_DT if frame_dt is None else frame_dt),
+                                 int(bool(use_cache)), C.byref(t)))
+        out = {k: int(getattr(t, k)) for k in self._TUNING_KEYS}
+        out["frame_ms"] = float(t.frame_ms)
+        return out
 
     def set_stencil_path(self, on: bool):
         check(lib().vtx_set_stencil_path(self.h, int(bool(on))))
