@@ -178,6 +178,15 @@ int vtx_read_inv_masses(vtx_solver* s, int32_t seg, float* w);
 int vtx_device_positions(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
 int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
 
+/* Extension (the reference has no checkpointing; "reset" rebuilds scenes, Velvet/VtEngine.cpp:85-107):
+ * the whole simulated state of a segment -- positions with inverse masses, velocities, initial
+ * positions, slot positions -- to and from a self-describing host blob. Restore needs a segment
+ * with the same particle and slot counts; registration data (constraints, colliders) is not in
+ * the blob. */
+int vtx_snapshot_size(vtx_solver* s, int32_t seg, uint64_t* bytes);
+int vtx_snapshot(vtx_solver* s, int32_t seg, void* buf, uint64_t bytes);
+int vtx_restore(vtx_solver* s, int32_t seg, const void* buf, uint64_t bytes);
+
 /* Host edit of one particle, the MouseGrabber effect (Velvet/MouseGrabber.hpp:51-52,60,74-75).
  * pos/vel may be NULL to keep them; inv_mass < 0 keeps it. */
 int vtx_set_particle(vtx_solver* s, int32_t seg, int32_t i, const float* pos, const float* vel,
@@ -249,6 +258,10 @@ int vtx_server_step(vtx_server* sv, float frame_dt, vtx_server_stats* out);
 int vtx_server_job_status(vtx_server* sv, int32_t job, int32_t* status, int32_t* frames_done);
 int vtx_server_read_result(vtx_server* sv, int32_t job, float* xyz, uint32_t n);
 int vtx_server_release_job(vtx_server* sv, int32_t job);
+/* Take a running job off the device (state kept on the host, device memory freed) and queue it
+ * again behind the waiting jobs; it resumes from the frame it reached with the same result as
+ * an uninterrupted run. */
+int vtx_server_preempt(vtx_server* sv, int32_t job);
 
 #ifdef __cplusplus
 }

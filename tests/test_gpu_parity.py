@@ -335,3 +335,38 @@ def test_dense_and_listed_attachments_agree(stencil):
     xa, va, _ = _run_gpu(sc, 4, stencil=stencil)
     xb, vb, _ = _run_gpu(sc2, 4, stencil=stencil)
     assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def test_snapshot_restore_roundtrip():
+    """A segment's snapshot taken mid-run replays bit-identically, in place and in a fresh solver
+    with the same registration; mismatched blobs are rejected."""
+    say("snapshot / restore")
+    scenes = S.config4(count=4, frames=6, smax=40)
+    with ClothSolver() as s:
+        for sc in scenes:
+            s.load_scene(sc)
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        snap = s.snapshot(seg=2)
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        xa, va = s.positions(2), s.velocities(2)
+        s.restore(snap, seg=2)
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        assert np.array_equal(s.positions(2), xa) and np.array_equal(s.velocities(2), va)
+        with pytest.raises(Exception):
+            s.restore(snap[:-16], seg=2)
+        if scenes[1].n != scenes[2].n:
+            with pytest.raises(Exception):
+                s.restore(snap, seg=1)
+    with ClothSolver() as s2:
+        s2.load_scene(scenes[2])
+        s2.restore(snap)
+        for _ in range(3):
+            s2.Simulate()
+        s2.synchronize()
+        assert np.array_equal(s2.positions(), xa) and np.array_equal(s2.velocities(), va)

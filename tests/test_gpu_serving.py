@@ -4,7 +4,9 @@
 * admission respects the device-memory budget and the live-particle cap, queues the rest and
   drains the queue as jobs finish; a job larger than the budget is rejected;
 * a job whose state turns non-finite (forced from the host) is evicted as failed while the
-  other jobs' results stay identical to their solo runs.
+  other jobs' results stay identical to their solo runs;
+* a preempted job (state parked on the host, device memory freed, queued again) ends
+  bit-identical to its solo run.
 """
 import sys
 
@@ -120,6 +122,30 @@ def test_step_time_budget_admission():
                 break
         assert most == len(js)      # one job on the first step (no measurement yet), then the rest
         assert all(sv.status(j)[0] == DONE for j in ids)
+
+
+def test_preempt_resume_equals_alone():
+    """A running job taken off the device and re-admitted later ends bit-identical to its solo run."""
+    say("serving preemption")
+    js = jobs(3)
+    with Server() as sv:
+        ids = [sv.submit(sc, f) for sc, f in js]
+        sv.step()
+        sv.step()
+        sv.preempt(ids[1])
+        status, done = sv.status(ids[1])
+        assert status == QUEUED and done == 2
+        st = sv.step()
+        assert st.admitted == 1
+        for _ in range(60):
+            sv.step()
+            if all(sv.status(j)[0] == DONE for j in ids):
+                break
+        for (sc, f), j in zip(js, ids):
+            assert sv.status(j) == (DONE, f)
+            assert np.array_equal(sv.result(j), alone(sc, f))
+        with pytest.raises(Exception):
+            sv.preempt(ids[0])          # done jobs cannot be preempted
 
 
 def test_nonfinite_job_is_evicted():

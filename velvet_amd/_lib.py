@@ -75,6 +75,9 @@ _SIGS = {
     "vtx_synchronize": (i32, [C.c_void_p]),
     "vtx_set_use_graph": (i32, [C.c_void_p, i32]),
     "vtx_get_tuning": (i32, [C.c_void_p, P(VtxTuning)]),
+    "vtx_snapshot_size": (i32, [C.c_void_p, i32, P(C.c_uint64)]),
+    "vtx_snapshot": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),
+    "vtx_restore": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),
     "vtx_set_tuning": (i32, [C.c_void_p, P(VtxTuning)]),
     "vtx_autotune": (i32, [C.c_void_p, i32, f32, i32, P(VtxTuning)]),
     "vtx_set_stream": (i32, [C.c_void_p, C.c_void_p]),
@@ -95,6 +98,7 @@ _SIGS = {
     "vtx_set_timing": (i32, [C.c_void_p, i32]),
     "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "vtx_server_create": (i32, [P(VtxServerConfig), P(C.c_void_p)]),
+    "vtx_server_preempt": (i32, [C.c_void_p, i32]),
     "vtx_server_destroy": (i32, [C.c_void_p]),
     "vtx_server_solver": (i32, [C.c_void_p, P(C.c_void_p)]),
     "vtx_server_new_job": (i32, [C.c_void_p, P(VtxParams), i32, P(i32), P(i32)]),

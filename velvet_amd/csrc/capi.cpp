@@ -289,6 +289,21 @@ int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n) {
     API_BEGIN *ptr = (void*)s->engine.device_normals(seg, n); API_END
 }
 
+int vtx_snapshot_size(vtx_solver* s, int32_t seg, uint64_t* bytes) {
+    NEED(s); NEED(bytes);
+    API_BEGIN *bytes = (uint64_t)s->engine.snapshot_bytes(seg); API_END
+}
+
+int vtx_snapshot(vtx_solver* s, int32_t seg, void* buf, uint64_t bytes) {
+    NEED(s); NEED(buf);
+    API_BEGIN s->engine.snapshot(seg, buf, (size_t)bytes); API_END
+}
+
+int vtx_restore(vtx_solver* s, int32_t seg, const void* buf, uint64_t bytes) {
+    NEED(s); NEED(buf);
+    API_BEGIN s->engine.restore(seg, buf, (size_t)bytes); API_END
+}
+
 int vtx_set_particle(vtx_solver* s, int32_t seg, int32_t i, const float* pos, const float* vel, float inv_mass) {
     NEED(s);
     API_BEGIN s->engine.set_particle(seg, i, pos, vel, inv_mass); API_END
@@ -381,6 +396,11 @@ int vtx_server_read_result(vtx_server* sv, int32_t job, float* xyz, uint32_t n) 
     if (n != j.n) throw vtx::Error(VTX_EINVAL, "particle count mismatch");
     std::memcpy(xyz, j.result.data(), sizeof(float) * 3 * (size_t)n);
     API_END
+}
+
+int vtx_server_preempt(vtx_server* sv, int32_t job) {
+    NEED(sv);
+    API_BEGIN sv->server.preempt(job); API_END
 }
 
 int vtx_server_release_job(vtx_server* sv, int32_t job) {

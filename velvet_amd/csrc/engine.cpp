@@ -16,6 +16,8 @@
 
 namespace vtx {
 
+static void fill_state(const Segment& sg, Prepared& pr);
+
 static const float kFixedDt = 1.0f / 60.0f;   // Velvet/Timer.hpp:235
 
 // ---------------------------------------------------------------------------- ranges
@@ -324,6 +326,98 @@ void Engine::set_active(int32_t s, bool active) {
     tables_dirty_ = true;
 }
 
+// Deactivate a segment and give its device memory back; its state goes to the host and its
+// prepared layout is kept, so re-activation only uploads.
+void Engine::park(int32_t s) {
+    Segment& sg = seg(s);
+    if (sg.uploaded) {
+        pull_state(sg);
+        if (sg.prepared) fill_state(sg, sg.prep);
+        release(sg);
+        if (sg.active) tables_dirty_ = true;
+    }
+    if (sg.active) {
+        sg.active = false;
+        tables_dirty_ = true;
+    }
+}
+
+namespace {
+// Snapshot blob: header, then n float4 positions (w = inverse mass), n float4 velocities,
+// n float4 initial positions and L float4 slot positions.
+struct SnapHeader {
+    uint32_t magic, version, n, nslots;
+    uint64_t bytes, reserved;
+};
+constexpr uint32_t kSnapMagic = 0x53585456u;   // "VTXS"
+}  // namespace
+
+size_t Engine::snapshot_bytes(int32_t s) {
+    Segment& sg = seg(s);
+    return sizeof(SnapHeader) + sizeof(float4) * (3 * (size_t)sg.n() + sg.slots.size() / 3);
+}
+
+void Engine::snapshot(int32_t s, void* buf, size_t bytes) {
+    Segment& sg = seg(s);
+    const size_t need = snapshot_bytes(s);
+    VTX_REQUIRE(buf && bytes >= need, VTX_EINVAL, "snapshot buffer too small");
+    const uint32_t n = sg.n(), L = (uint32_t)(sg.slots.size() / 3);
+    SnapHeader h{kSnapMagic, 1u, n, L, (uint64_t)need, 0};
+    std::memcpy(buf, &h, sizeof h);
+    float4* out = reinterpret_cast<float4*>(static_cast<char*>(buf) + sizeof h);
+    if (sg.uploaded) {
+        synchronize();
+        if (n) {
+            VTX_HIP(hipMemcpy(out, pl_.X + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+            VTX_HIP(hipMemcpy(out + n, pl_.V + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+            VTX_HIP(hipMemcpy(out + 2 * (size_t)n, pl_.X0 + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        }
+        if (L) VTX_HIP(hipMemcpy(out + 3 * (size_t)n, pl_.slots + sg.rg.slot, sizeof(float4) * L, hipMemcpyDeviceToHost));
+    } else {
+        Prepared tmp;
+        fill_state(sg, tmp);
+        std::copy(tmp.x.begin(), tmp.x.end(), out);
+        std::copy(tmp.v.begin(), tmp.v.end(), out + n);
+        std::copy(tmp.x0.begin(), tmp.x0.end(), out + 2 * (size_t)n);
+        std::copy(tmp.slots.begin(), tmp.slots.end(), out + 3 * (size_t)n);
+    }
+}
+
+void Engine::restore(int32_t s, const void* buf, size_t bytes) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(buf && bytes >= sizeof(SnapHeader), VTX_EINVAL, "snapshot buffer too small");
+    SnapHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    const uint32_t n = sg.n(), L = (uint32_t)(sg.slots.size() / 3);
+    VTX_REQUIRE(h.magic == kSnapMagic && h.version == 1u, VTX_EINVAL, "not a segment snapshot");
+    VTX_REQUIRE(h.n == n && h.nslots == L, VTX_EINVAL, "snapshot is of a segment with other particle or slot counts");
+    VTX_REQUIRE(h.bytes == snapshot_bytes(s) && bytes >= h.bytes, VTX_EINVAL, "snapshot size mismatch");
+    const float4* in = reinterpret_cast<const float4*>(static_cast<const char*>(buf) + sizeof h);
+    for (uint32_t i = 0; i < n; i++) {
+        const float4 x = in[i], v = in[n + i], x0 = in[2 * (size_t)n + i];
+        sg.x[3 * i] = x.x; sg.x[3 * i + 1] = x.y; sg.x[3 * i + 2] = x.z;
+        sg.w[i] = x.w;
+        sg.v[3 * i] = v.x; sg.v[3 * i + 1] = v.y; sg.v[3 * i + 2] = v.z;
+        sg.x0[3 * i] = x0.x; sg.x0[3 * i + 1] = x0.y; sg.x0[3 * i + 2] = x0.z;
+    }
+    for (uint32_t k = 0; k < L; k++) {
+        const float4 q = in[3 * (size_t)n + k];
+        sg.slots[3 * k] = q.x; sg.slots[3 * k + 1] = q.y; sg.slots[3 * k + 2] = q.z;
+    }
+    if (sg.prepared) fill_state(sg, sg.prep);
+    if (sg.uploaded) {
+        synchronize();
+        if (n) {
+            VTX_HIP(hipMemcpy(pl_.X + sg.rg.p, in, sizeof(float4) * n, hipMemcpyHostToDevice));
+            VTX_HIP(hipMemcpy(pl_.V + sg.rg.p, in + n, sizeof(float4) * n, hipMemcpyHostToDevice));
+            VTX_HIP(hipMemcpy(pl_.X0 + sg.rg.p, in + 2 * (size_t)n, sizeof(float4) * n, hipMemcpyHostToDevice));
+            VTX_HIP(hipMemcpy(pl_.P0 + sg.rg.p, in, sizeof(float4) * n, hipMemcpyHostToDevice));
+            VTX_HIP(hipMemcpy(pl_.P1 + sg.rg.p, in, sizeof(float4) * n, hipMemcpyHostToDevice));
+        }
+        if (L) VTX_HIP(hipMemcpy(pl_.slots + sg.rg.slot, in + 3 * (size_t)n, sizeof(float4) * L, hipMemcpyHostToDevice));
+    }
+}
+
 void Engine::invalidate_graph() {
     if (gexec_) {
         (void)hipGraphExecDestroy(gexec_);
@@ -486,14 +580,10 @@ void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------- preparation
-// Host packing of one segment into its position-independent device layout.
-void Engine::prepare(int32_t si) {
-    Segment& sg = seg(si);
-    if (sg.prepared) return;
-    Prepared pr;
+// The simulated state of a segment in device format: positions with inverse masses,
+// velocities, initial positions and slot positions.
+static void fill_state(const Segment& sg, Prepared& pr) {
     const uint32_t n = sg.n();
-    VTX_REQUIRE(!sg.selfcol() || n < (1u << 25), VTX_ECAPACITY,
-                "self collision supports at most 2^25 particles per segment");
     pr.x.resize(n);
     pr.v.resize(n);
     pr.x0.resize(n);
@@ -503,8 +593,21 @@ void Engine::prepare(int32_t si) {
         pr.x0[i] = make_float4(sg.x0[3 * i], sg.x0[3 * i + 1], sg.x0[3 * i + 2], 0.f);
     }
     const uint32_t L = (uint32_t)(sg.slots.size() / 3);
+    pr.slots.resize(L);
     for (uint32_t k = 0; k < L; k++)
-        pr.slots.push_back(make_float4(sg.slots[3 * k], sg.slots[3 * k + 1], sg.slots[3 * k + 2], 0.f));
+        pr.slots[k] = make_float4(sg.slots[3 * k], sg.slots[3 * k + 1], sg.slots[3 * k + 2], 0.f);
+}
+
+// Host packing of one segment into its position-independent device layout.
+void Engine::prepare(int32_t si) {
+    Segment& sg = seg(si);
+    if (sg.prepared) return;
+    Prepared pr;
+    const uint32_t n = sg.n();
+    VTX_REQUIRE(!sg.selfcol() || n < (1u << 25), VTX_ECAPACITY,
+                "self collision supports at most 2^25 particles per segment");
+    fill_state(sg, pr);
+    const uint32_t L = (uint32_t)(sg.slots.size() / 3);
     for (auto& c : sg.cols) pr.cols.push_back(pack_collider(c));
 
     GridLayout G;

@@ -120,6 +120,12 @@ public:
     int32_t add_segment(const vtx_params& p, bool active = true);
     void remove_segment(int32_t s);
     void set_active(int32_t s, bool active);
+    void park(int32_t s);                    // deactivate and free its device memory, keep its state
+    // Whole simulated state of a segment (positions + inverse masses, velocities, initial
+    // positions, slot positions) to / from a self-describing host blob.
+    size_t snapshot_bytes(int32_t s);
+    void snapshot(int32_t s, void* buf, size_t bytes);
+    void restore(int32_t s, const void* buf, size_t bytes);
     int32_t num_segments() const;
     Segment& seg(int32_t i);
 

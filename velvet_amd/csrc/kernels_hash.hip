@@ -248,12 +248,18 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 //                 their loads merge;
 //   NBR_PATCH:    16 x 16 vertex patches of a grid cloth (its stencil tiles): a wave is 4 rows x
 //                 16 columns, so lanes share cells and candidates, and each row of the
-//                 column-major table is written as four 64-byte runs. A first sweep over the 27 cells finds the non-empty ones (bit q of
-// mask) and the total candidate count; the walk is one flat loop over their concatenation in
-// the same order, reloading a cell's range when it gets there (an L1 hit).
+//                 column-major table is written as four 64-byte runs.
+// A first sweep over the 27 cells finds the non-empty ones (bit q of mask) and the total
+// candidate count, and keeps the ranges of the first VTX_NB_CACHE non-empty cells in this
+// lane's LDS column, packed as start | length << 25 (self-collision segments have < 2^25
+// particles, lengths are capped at max_nbr <= 64); a lane with more non-empty cells recomputes
+// the rest. The walk is one flat loop over the cells' concatenation in the same order, with the
+// next candidate's position and initial position loaded while the current one is tested.
+#define VTX_NB_CACHE 16
 template <int ORDER>
 __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t crange[VTX_NB_CACHE][VTX_TILE];   // lane-private columns: no barrier needed
     uint32_t seg, i;
     if (ORDER == NBR_PATCH) {
         const GridTileDev t = d.gtiles[tile_of(d.remap)];
@@ -282,33 +288,58 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDe
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
     const uint64_t mc = seg_mod(sg);
     const uint32_t K = sg.max_nbr;
-    uint32_t mask = 0, total = 0;
+    uint32_t mask = 0, total = 0, nz = 0;
 #pragma unroll
     for (int q = 0; q < 27; q++) {
         const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
         const uint32_t len = c.x == VTX_INVALID ? 0u : min(c.y, c.x + K) - c.x;
+        if (len && nz < VTX_NB_CACHE) crange[nz][threadIdx.x] = c.x | len << 25;
+        nz += len ? 1u : 0u;
         mask |= (len ? 1u : 0u) << q;
         total += len;
     }
-    uint32_t k = 0, s = 0, left = 0;
-    for (uint32_t it = 0; it < total; it++) {
-        if (left == 0) {         // next non-empty cell: one exists while it < total
+    if (total == 0) {
+        nb[0] = VTX_INVALID;
+        return;
+    }
+    // position of the next candidate (called only while candidates remain)
+    uint32_t z = 0, s = 0, left = 0;
+    auto next = [&]() -> uint32_t {
+        if (left == 0) {
             const int q = __builtin_ctz(mask);
             mask &= mask - 1;
-            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
-            s = c.x;
-            left = min(c.y, c.x + K) - c.x;
+            uint32_t st, ln;
+            if (z < VTX_NB_CACHE) {
+                const uint32_t e = crange[z][threadIdx.x];
+                st = e & 0x1FFFFFFu;
+                ln = e >> 25;
+            } else {
+                const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
+                st = c.x;
+                ln = min(c.y, c.x + K) - c.x;
+            }
+            z++;
+            s = st;
+            left = ln;
         }
-        const uint32_t cur = s;
-        s++;
         left--;
-        const float4 qp = sP[cur];
+        return s++;
+    };
+    uint32_t cur = next();
+    float4 qn = sP[cur], rn = sX0[cur];
+    uint32_t k = 0;
+    for (uint32_t it = 0; it < total; it++) {
+        const float4 qp = qn, r = rn;
+        if (it + 1 < total) {
+            cur = next();
+            qn = sP[cur];
+            rn = sX0[cur];
+        }
         const uint32_t j = __float_as_uint(qp.w);
         if (j == id) continue;                               // the particle itself
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         if (!(d2 < sg.cell2)) continue;
-        const float4 r = sX0[cur];
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         if (!(e2 > sg.diam2)) continue;

@@ -134,6 +134,21 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
     if (out) *out = st;
 }
 
+// A running job leaves the device (its state parked on the host, its device ranges freed) and
+// queues again behind the jobs already waiting; when re-admitted it continues from the frame it
+// reached, with the same result as an uninterrupted run.
+void Server::preempt(int32_t id) {
+    Job& j = get(id);
+    VTX_REQUIRE(j.status == VTX_JOB_RUNNING, VTX_ESTATE, "only a running job can be preempted");
+    e_.park(j.seg);
+    live_.erase(std::find(live_.begin(), live_.end(), id));
+    resident_ -= j.bytes;
+    live_particles_ -= j.n;
+    live_work_ -= j.work;
+    j.status = VTX_JOB_QUEUED;
+    queue_.push_back(id);
+}
+
 void Server::release(int32_t id) {
     Job& j = get(id);
     VTX_REQUIRE(j.status != VTX_JOB_RUNNING && j.status != VTX_JOB_QUEUED, VTX_ESTATE,

@@ -2,6 +2,7 @@
 // horizon) queue up, are admitted while device memory and the live-particle budget allow, all
 // admitted jobs advance one frame per step in the same launches, and finished jobs leave.
 #pragma once
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -29,6 +30,7 @@ public:
     void step(float frame_dt, vtx_server_stats* out);
     const Job& job(int32_t id) const;
     void release(int32_t id);
+    void preempt(int32_t id);
 
 private:
     Job& get(int32_t id);

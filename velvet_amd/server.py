@@ -79,6 +79,9 @@ class Server:
                                            self.n[job]))
         return out
 
+    def preempt(self, job: int):
+        check(lib().vtx_server_preempt(self.h, job))
+
     def release(self, job: int):
         check(lib().vtx_server_release_job(self.h, job))
         self.n.pop(job, None)

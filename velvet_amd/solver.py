@@ -208,6 +208,18 @@ class ClothSolver:
         out["frame_ms"] = float(t.frame_ms)
         return out
 
+    def snapshot(self, seg: int = 0) -> bytes:
+        """Positions (+ inverse masses), velocities, initial and slot positions of a segment."""
+        n = C.c_uint64()
+        check(lib().vtx_snapshot_size(self.h, seg, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().vtx_snapshot(self.h, seg, buf, n.value))
+        return buf.raw
+
+    def restore(self, blob: bytes, seg: int = 0):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        check(lib().vtx_restore(self.h, seg, buf, len(blob)))
+
     def set_stencil_path(self, on: bool):
         check(lib().vtx_set_stencil_path(self.h, int(bool(on))))
 
