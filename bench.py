@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--tune-frames", type=int, default=3,
                     help="frames per variant for the launch-variant tuner before warm-up; 0 = defaults")
+    ap.add_argument("--set-tuning", default="", metavar="K=V,...",
+                    help="fix launch variants instead of tuning, e.g. use_graph=1,neighbor_order=1")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-rates", default="50", help="comma-separated Poisson rates (req/s)")
@@ -242,7 +244,10 @@ def main():
     for sc in scenes:
         solver.load_scene(sc)
     tuning = solver.tuning()
-    if args.tune_frames > 0:
+    if args.set_tuning:
+        solver.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.set_tuning.split(","))})
+        tuning = solver.tuning()
+    elif args.tune_frames > 0:
         log(f"tuning launch variants ({args.tune_frames} frames each)")
         tuning = solver.autotune(frames=args.tune_frames)
         log(f"  {tuning}")

@@ -149,7 +149,7 @@ typedef struct vtx_tuning {
     int32_t use_graph;         /* replay frames from a hipGraph */
     int32_t neighbor_order;    /* neighbour search thread order: 0 particle index, 1 hash-sorted,
                                   2 vertex patches for grid cloths (sorted for other meshes) */
-    int32_t lra_batch;         /* dense attachment distances loaded per batch: 4 or 16 */
+    int32_t lra_batch;         /* dense attachment distances loaded per batch: 4, 16 or 64 */
     int32_t xcd_remap;         /* hand each XCD a contiguous run of tiles */
     float frame_ms;            /* vtx_autotune: the chosen variant's frame time (0 from the cache) */
     int32_t pad;

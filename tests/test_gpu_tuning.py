@@ -53,7 +53,7 @@ def _run(scenes, frames, **tuning):
 def test_variants_bit_identical(which):
     scenes = _batch() if which == "grid batch" else _general()
     ref = _run(scenes, 4)
-    for g, so, lb, rm in itertools.product((0, 1), (0, 1, 2), (4, 16), (0, 1)):
+    for g, so, lb, rm in itertools.product((0, 1), (0, 1, 2), (4, 16, 64), (0, 1)):
         say(f"{which}: graph={g} neighbor_order={so} lra_batch={lb} remap={rm}")
         got = _run(scenes, 4, use_graph=g, neighbor_order=so, lra_batch=lb, xcd_remap=rm)
         for k, ((x, v), (xr, vr)) in enumerate(zip(got, ref)):
@@ -64,8 +64,9 @@ def test_variants_bit_identical_long_range_attachments():
     say("variants on config 2 (256 dense slots)")
     sc = S.config2(frames=2)
     ref = _run([sc], 2)
-    got = _run([sc], 2, lra_batch=4, neighbor_order=0, xcd_remap=0)
-    assert np.array_equal(got[0][0], ref[0][0]) and np.array_equal(got[0][1], ref[0][1])
+    for lb in (4, 64):
+        got = _run([sc], 2, lra_batch=lb, neighbor_order=0, xcd_remap=0)
+        assert np.array_equal(got[0][0], ref[0][0]) and np.array_equal(got[0][1], ref[0][1])
 
 
 def test_autotune_keeps_state_and_caches():

@@ -1279,7 +1279,8 @@ void Engine::set_use_graph(bool on) {
 }
 
 void Engine::set_tuning(const Tuning& t) {
-    VTX_REQUIRE(t.lra_batch == 4 || t.lra_batch == 16, VTX_EINVAL, "lra_batch must be 4 or 16");
+    VTX_REQUIRE(t.lra_batch == 4 || t.lra_batch == 16 || t.lra_batch == 64, VTX_EINVAL,
+                "lra_batch must be 4, 16 or 64");
     VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
@@ -1371,13 +1372,13 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         for (int knob = 0; knob < 4; knob++) {
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
-            const int nval = knob == 1 ? 3 : 2;
+            const int nval = knob == 1 || knob == 2 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
                 Tuning t = base;
                 if (knob == 0) t.graph = v != 0;
                 if (knob == 1) t.nbr_order = v;
-                if (knob == 2) t.lra_batch = v ? 16 : 4;
+                if (knob == 2) t.lra_batch = v == 0 ? 4 : v == 1 ? 16 : 64;
                 if (knob == 3) t.remap = v != 0;
                 if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
                                 : knob == 2 ? t.lra_batch == base.lra_batch : t.remap == base.remap)

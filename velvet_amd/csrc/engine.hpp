@@ -101,7 +101,7 @@ struct Segment {
 struct Tuning {
     bool graph = true;        // replay frames from a hipGraph
     int nbr_order = 2;        // neighbour search thread order: 0 particle, 1 sorted, 2 grid patches
-    int lra_batch = 16;       // dense attachment distances loaded per batch: 4 or 16
+    int lra_batch = 16;       // dense attachment distances loaded per batch: 4, 16 or 64
     bool remap = true;        // XCD-aware workgroup -> tile remap
 };
 

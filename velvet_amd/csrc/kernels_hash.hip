@@ -256,6 +256,13 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // the rest. The walk is one flat loop over the cells' concatenation in the same order, with the
 // next candidate's position and initial position loaded while the current one is tested.
 #define VTX_NB_CACHE 16
+
+// Element i < 2^25 of a float4 array whose base is wave-uniform: a 32-bit byte offset lets the
+// load use the scalar-base + 32-bit vector-offset form.
+__device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (size_t)((i & 0x1FFFFFFu) << 4));
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
 #pragma clang fp contract(off)
@@ -325,26 +332,39 @@ __global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDe
         left--;
         return s++;
     };
-    uint32_t cur = next();
-    float4 qn = sP[cur], rn = sX0[cur];
+    // candidate test in walk order; true once the list is full
     uint32_t k = 0;
-    for (uint32_t it = 0; it < total; it++) {
-        const float4 qp = qn, r = rn;
-        if (it + 1 < total) {
-            cur = next();
-            qn = sP[cur];
-            rn = sX0[cur];
-        }
+    auto take = [&](float4 qp, float4 r) -> bool {
         const uint32_t j = __float_as_uint(qp.w);
-        if (j == id) continue;                               // the particle itself
+        if (j == id) return false;                           // the particle itself
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        if (!(d2 < sg.cell2)) continue;
+        if (!(d2 < sg.cell2)) return false;
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-        if (!(e2 > sg.diam2)) continue;
+        if (!(e2 > sg.diam2)) return false;
         nb[(size_t)k * n] = j;
-        if (++k >= K) return;                                // full: no sentinel
+        return ++k >= K;
+    };
+    // two candidates per trip with alternating load buffers: the loads of the next candidate
+    // are in flight while the current one is tested
+    uint32_t cur = next();
+    float4 qa = ld_sorted(sP, cur), ra = ld_sorted(sX0, cur), qb, rb;
+    for (uint32_t it = 0; it < total; it += 2) {
+        const bool two = it + 1 < total;
+        if (two) {
+            cur = next();
+            qb = ld_sorted(sP, cur);
+            rb = ld_sorted(sX0, cur);
+        }
+        if (take(qa, ra)) return;                            // full: no sentinel
+        if (!two) break;
+        if (it + 2 < total) {
+            cur = next();
+            qa = ld_sorted(sP, cur);
+            ra = ld_sorted(sX0, cur);
+        }
+        if (take(qb, rb)) return;
     }
     nb[(size_t)k * n] = VTX_INVALID;
 }

@@ -594,7 +594,8 @@ hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool
         else hipLaunchKernelGGL((k_iterate<EPI_FINALIZE_PREDICT, false, 1>), gr, bl, 0, s, d, d.tiles, in, h);
         return last_launch();
     }
-    if (lra_batch >= 16) launch_iterate_lb<16>(d, in, h, epilogue, s);
+    if (lra_batch >= 64) launch_iterate_lb<64>(d, in, h, epilogue, s);
+    else if (lra_batch >= 16) launch_iterate_lb<16>(d, in, h, epilogue, s);
     else launch_iterate_lb<4>(d, in, h, epilogue, s);
     return last_launch();
 }

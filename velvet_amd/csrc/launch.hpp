@@ -10,7 +10,7 @@ enum { EPILOGUE_NONE = 0, EPILOGUE_FINALIZE = 1, EPILOGUE_FINALIZE_PREDICT = 2 }
 
 hipError_t launch_frame_begin(const DevState& d, float frame_dt, float h, hipStream_t s);
 hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s);
-// lra_batch: dense attachment distances loaded per batch (4 or 16; same results)
+// lra_batch: dense attachment distances loaded per batch (4, 16 or 64; same results)
 hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints, int lra_batch,
                           hipStream_t s);
 hipError_t launch_normals(const DevState& d, hipStream_t s);
