@@ -133,6 +133,20 @@ def plain_rate(scenes, frames, stream):
     return dt
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the plain Jacobi pass from the committed, copy-calibrated
+    FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_counters.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("| `void k_iterate_grid<0,"):
+                cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+                return (float(cells[1]) + float(cells[2])) * 1e6, os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def cpu_baseline(scenes, budget_s):
     import oracle
     done_ps, spent, frames, cloths = 0, 0.0, 0, 0
@@ -267,9 +281,10 @@ def main():
     avg_ms = it["ms"] / max(1, it["launches"])
     alg_bytes = S.algorithmic_bytes_per_iteration(tot)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_iterate", "avg_launch_ms": round(avg_ms, 6),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src, "kernel": "k_iterate", "avg_launch_ms": round(avg_ms, 6),
                 "algorithmic_bytes_per_launch": alg_bytes}
     solver.close()
 

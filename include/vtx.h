@@ -178,6 +178,21 @@ int vtx_read_inv_masses(vtx_solver* s, int32_t seg, float* w);
 int vtx_device_positions(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
 int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n);
 
+/* Extension: constraints of an arbitrary triangle mesh. The reference generates them from grid
+ * indices only, with bending rest angle 0 (Velvet/VtClothObjectGPU.hpp:75-132, TODO at 128-129).
+ * Stretch: every unique edge, rest = current length, as index pairs. Bending: every edge of
+ * exactly two triangles, as (p0, p1, p2, p3) = (third vertex of the first triangle, third vertex
+ * of the second, the edge in the first triangle's winding), rest = the current angle between the
+ * two triangle normals, so the given shape is at rest. Edges in order of first appearance.
+ * vtx_mesh_constraints is host only (no solver, no GPU): counts are always written; arrays are
+ * written when non-NULL and must hold stretch_cap pairs / bend_cap quadruples (else
+ * VTX_ECAPACITY). vtx_add_mesh_constraints generates them from the current positions of a cloth
+ * added at `offset` (triangle indices local to that cloth) and registers them on `seg`. */
+int vtx_mesh_constraints(const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
+                         int32_t* stretch_ij, float* stretch_rest, uint32_t stretch_cap, uint32_t* stretch_count,
+                         uint32_t* bend_idx, float* bend_rest, uint32_t bend_cap, uint32_t* bend_count);
+int vtx_add_mesh_constraints(vtx_solver* s, int32_t seg, int32_t offset, const uint32_t* tris, uint32_t ntri);
+
 /* Extension (the reference has no checkpointing; "reset" rebuilds scenes, Velvet/VtEngine.cpp:85-107):
  * the whole simulated state of a segment -- positions with inverse masses, velocities, initial
  * positions, slot positions -- to and from a self-describing host blob. Restore needs a segment

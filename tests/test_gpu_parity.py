@@ -370,3 +370,67 @@ def test_snapshot_restore_roundtrip():
             s2.Simulate()
         s2.synchronize()
         assert np.array_equal(s2.positions(), xa) and np.array_equal(s2.velocities(), va)
+
+
+def _irregular_horizontal(res, seed, y=1.4, scale=0.5):
+    x, tris = S.irregular_mesh(res, seed=seed, stream=11)
+    h = np.empty_like(x)
+    h[:, 0] = x[:, 0] * scale
+    h[:, 1] = y
+    h[:, 2] = (-x[:, 1] - 1.0) * scale
+    return h.astype(F32), tris
+
+
+def test_mesh_rest_angles_hold_a_curved_shape():
+    """An irregular mesh rolled into a cylinder arc keeps its shape with generated rest angles
+    (no gravity, nothing touching it); with the rest angles zeroed it unrolls."""
+    say("mesh rest angles")
+    x, tris = _irregular_horizontal(20, seed=4)
+    R = 0.4
+    th = x[:, 0] / R
+    curved = np.stack([R * np.sin(th), 1.4 + R * (np.cos(th) - 1.0), x[:, 2]], axis=1).astype(F32)
+    spacing = 1.0 / 20
+    c = S.mesh_cloth(curved, tris, diameter=spacing * 1.5)
+    assert c.bend.shape[0] > 0 and float(np.max(c.bend_rest)) > 0.05
+    p = S.Params(num_substeps=5, num_iterations=10, gravity=(0.0, 0.0, 0.0))
+    sc = S.Scene(S.finish_params(p, [c]), [c], [], 20)
+    xg, _, _ = _run_gpu(sc, 20, stencil=True, expect_stencil=0)
+    moved = float(np.max(np.abs(xg - curved)))
+    flat = S.Cloth(**{**c.__dict__, "bend_rest": np.zeros_like(c.bend_rest)})
+    sc0 = S.Scene(sc.params, [flat], [], 20)
+    x0, _, _ = _run_gpu(sc0, 20)
+    moved0 = float(np.max(np.abs(x0 - curved)))
+    say(f"  moved {moved:.2e} with rest angles, {moved0:.2e} without")
+    assert moved <= 1e-4 * sc.extent()
+    assert moved0 > 100 * moved
+
+
+def test_irregular_mesh_vs_fp64():
+    say("irregular mesh vs fp64")
+    x, tris = _irregular_horizontal(24, seed=2)
+    c = S.mesh_cloth(x, tris, diameter=1.5 / 24 * 0.5, attached=(0, 24))
+    p = S.Params(num_substeps=5, num_iterations=10)
+    sc = S.Scene(S.finish_params(p, [c]), [c], [S.sphere((0.0, 0.9, -0.5), 0.35)], 10)
+    xg, _, _ = _run_gpu(sc, 10)
+    ref = oracle.simulate(sc, frames=10).x.reshape(-1, 3)
+    err = float(np.max(np.abs(xg.astype(np.float64) - ref)))
+    say(f"  max err {err:.3e}")
+    assert err <= 1e-3 * sc.extent()
+
+
+def test_add_mesh_constraints_matches_registered_arrays():
+    say("add_mesh_constraints")
+    x, tris = _irregular_horizontal(16, seed=6)
+    c = S.mesh_cloth(x, tris, diameter=0.05)
+    p = S.finish_params(S.Params(num_substeps=3, num_iterations=6), [c])
+    sc = S.Scene(p, [c], [S.sphere((0.0, 1.0, -0.5), 0.3)], 5)
+    xa, va, _ = _run_gpu(sc, 5)
+    with ClothSolver() as s:
+        s.set_params(p)
+        off = s.AddCloth(x, tris, diameter=float(c.diameter))
+        s.add_mesh_constraints(off, tris)
+        s.UpdateColliders(sc.colliders)
+        for _ in range(5):
+            s.Simulate()
+        s.synchronize()
+        assert np.array_equal(s.positions(), xa) and np.array_equal(s.velocities(), va)

@@ -75,6 +75,9 @@ _SIGS = {
     "vtx_synchronize": (i32, [C.c_void_p]),
     "vtx_set_use_graph": (i32, [C.c_void_p, i32]),
     "vtx_get_tuning": (i32, [C.c_void_p, P(VtxTuning)]),
+    "vtx_mesh_constraints": (i32, [P(C.c_float), u32, P(u32), u32, P(i32), P(C.c_float), u32, P(u32),
+                                   P(u32), P(C.c_float), u32, P(u32)]),
+    "vtx_add_mesh_constraints": (i32, [C.c_void_p, i32, i32, P(u32), u32]),
     "vtx_snapshot_size": (i32, [C.c_void_p, i32, P(C.c_uint64)]),
     "vtx_snapshot": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),
     "vtx_restore": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),

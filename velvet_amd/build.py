@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvtx.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels_solver.hip", "kernels_hash.hip", "engine.cpp", "server.cpp", "capi.cpp"]
+SOURCES = ["kernels_solver.hip", "kernels_hash.hip", "engine.cpp", "server.cpp", "mesh.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VTX_ARCH", "gfx950")
 

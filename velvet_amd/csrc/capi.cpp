@@ -289,6 +289,35 @@ int vtx_device_normals(vtx_solver* s, int32_t seg, void** ptr, uint32_t* n) {
     API_BEGIN *ptr = (void*)s->engine.device_normals(seg, n); API_END
 }
 
+int vtx_mesh_constraints(const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
+                         int32_t* stretch_ij, float* stretch_rest, uint32_t stretch_cap, uint32_t* stretch_count,
+                         uint32_t* bend_idx, float* bend_rest, uint32_t bend_cap, uint32_t* bend_count) {
+    NEED(stretch_count); NEED(bend_count);
+    API_BEGIN
+    std::vector<int32_t> ij;
+    std::vector<float> sr, br;
+    std::vector<uint32_t> bi;
+    vtx::mesh_constraints(xyz, n, tris, ntri, ij, sr, bi, br);
+    *stretch_count = (uint32_t)sr.size();
+    *bend_count = (uint32_t)br.size();
+    if (stretch_ij || stretch_rest) {
+        VTX_REQUIRE(stretch_cap >= sr.size(), VTX_ECAPACITY, "stretch arrays too small");
+        if (stretch_ij) std::copy(ij.begin(), ij.end(), stretch_ij);
+        if (stretch_rest) std::copy(sr.begin(), sr.end(), stretch_rest);
+    }
+    if (bend_idx || bend_rest) {
+        VTX_REQUIRE(bend_cap >= br.size(), VTX_ECAPACITY, "bend arrays too small");
+        if (bend_idx) std::copy(bi.begin(), bi.end(), bend_idx);
+        if (bend_rest) std::copy(br.begin(), br.end(), bend_rest);
+    }
+    API_END
+}
+
+int vtx_add_mesh_constraints(vtx_solver* s, int32_t seg, int32_t offset, const uint32_t* tris, uint32_t ntri) {
+    NEED(s);
+    API_BEGIN s->engine.add_mesh_constraints(seg, offset, tris, ntri); API_END
+}
+
 int vtx_snapshot_size(vtx_solver* s, int32_t seg, uint64_t* bytes) {
     NEED(s); NEED(bytes);
     API_BEGIN *bytes = (uint64_t)s->engine.snapshot_bytes(seg); API_END

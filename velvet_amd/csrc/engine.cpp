@@ -560,6 +560,24 @@ void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32
     sg.bd_rest.insert(sg.bd_rest.end(), angle, angle + count);
 }
 
+void Engine::add_mesh_constraints(int32_t s, int32_t offset, const uint32_t* tris, uint32_t ntri) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(ntri == 0 || tris, VTX_EINVAL, "triangles are NULL");
+    VTX_REQUIRE(offset >= 0 && (uint32_t)offset <= sg.n(), VTX_EINVAL, "cloth offset out of range");
+    if (sg.uploaded) pull_state(sg);
+    std::vector<uint32_t> g((size_t)3 * ntri);
+    for (size_t t = 0; t < g.size(); t++) {
+        VTX_REQUIRE((uint64_t)tris[t] + (uint64_t)offset < sg.n(), VTX_EINVAL, "triangle index out of range");
+        g[t] = tris[t] + (uint32_t)offset;
+    }
+    std::vector<int32_t> ij;
+    std::vector<float> sr, br;
+    std::vector<uint32_t> bi;
+    mesh_constraints(sg.x.data(), sg.n(), g.data(), ntri, ij, sr, bi, br);
+    if (!sr.empty()) add_stretch(s, ij.data(), sr.data(), (uint32_t)sr.size());
+    if (!br.empty()) add_bend(s, bi.data(), br.data(), (uint32_t)br.size());
+}
+
 void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
     Segment& sg = seg(s);
     VTX_REQUIRE(n == 0 || cols, VTX_EINVAL, "colliders are NULL");

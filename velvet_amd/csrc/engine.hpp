@@ -110,6 +110,12 @@ struct StageTime {
     hipEvent_t a, b;
 };
 
+// Stretch (every unique edge) and bending (every edge of exactly two triangles, rest angle from
+// the given positions) constraints of a triangle mesh; host only (mesh.cpp).
+void mesh_constraints(const float* x, uint32_t n, const uint32_t* tris, uint32_t ntri,
+                      std::vector<int32_t>& st_ij, std::vector<float>& st_rest,
+                      std::vector<uint32_t>& bd_idx, std::vector<float>& bd_rest);
+
 class Engine {
 public:
     explicit Engine(const vtx_params& p);
@@ -137,6 +143,8 @@ public:
     void set_attach_slot_position(int32_t s, int32_t slot, const float* pos);
     void add_attach(int32_t s, const int32_t* pid, const int32_t* slot, const float* dist, uint32_t count);
     void add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count);
+    // Stretch and bending constraints of the cloth added at `offset`, from its current positions.
+    void add_mesh_constraints(int32_t s, int32_t offset, const uint32_t* tris, uint32_t ntri);
     void set_colliders(int32_t s, const vtx_collider* cols, uint32_t n);
 
     void prepare(int32_t s);                 // host packing ahead of upload (optional)

@@ -317,6 +317,89 @@ def grid_cloth(n: int, spacing: float, center=(0.0, 1.5, 0.0), jitter: float = 1
                  diameter=diam, side=n)
 
 
+def mesh_constraints(x: np.ndarray, tris: np.ndarray):
+    """Stretch (every unique edge) and bending (every edge of two triangles, rest angle from ``x``)
+    of a triangle mesh, from the library's generator (``vtx_mesh_constraints``).
+    Returns (stretch (S, 2) int32, stretch_rest (S,) float32, bend (B, 4) uint32, bend_rest (B,))."""
+    import ctypes as C
+    from ._lib import check, lib
+    x = np.ascontiguousarray(x, dtype=F32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    ns, nb = C.c_uint32(), C.c_uint32()
+    check(lib().vtx_mesh_constraints(fp(x), x.shape[0], up(t), t.shape[0], None, None, 0, C.byref(ns),
+                                     None, None, 0, C.byref(nb)))
+    st = np.empty((ns.value, 2), np.int32)
+    sr = np.empty(ns.value, F32)
+    bd = np.empty((nb.value, 4), np.uint32)
+    br = np.empty(nb.value, F32)
+    check(lib().vtx_mesh_constraints(fp(x), x.shape[0], up(t), t.shape[0], ip(st), fp(sr), ns.value,
+                                     C.byref(ns), up(bd), fp(br), nb.value, C.byref(nb)))
+    return st, sr, bd, br
+
+
+def irregular_mesh(res: int, seed: int = 0, stream: int = 0):
+    """GenerateClothMeshIrregular (Velvet/Scene.hpp:170-248): a (res+1)^2 grid of size 2.0 in the
+    xy plane, interior vertices moved by U[0, 0.4/res)^2, each quad split along the diagonal the
+    reference's angle test picks. Kept as in the reference: vertices are pushed y-major while
+    triangles index x * (res+1) + y, and the test compares acos(dot) of unnormalised edge vectors.
+    The reference draws its noise with rand(); the draws here are SplitMix64(seed, stream)."""
+    m = res + 1
+    noise = uniform(seed, stream, 2 * m * m, 0.0, 1.0).astype(F32).reshape(m, m, 2)   # [y, x]
+    ns = F32(1.0 / res * 0.4)
+    x = np.zeros((m * m, 3), dtype=F32)
+    k = 0
+    for yy in range(m):
+        for xx in range(m):
+            bnd = xx == 0 or yy == 0 or xx == res or yy == res
+            nz = np.zeros(2, F32) if bnd else ns * noise[yy, xx]
+            u = nz[0] + F32(xx) / F32(res)
+            v = nz[1] + F32(yy) / F32(res)
+            x[k] = F32(2.0) * np.array([u - F32(0.5), -v, 0.0], dtype=F32)
+            k += 1
+
+    def at(xx, yy):
+        return xx * m + yy
+
+    def angle(l, mid, r):
+        a, b = (l - mid).astype(F32), (r - mid).astype(F32)
+        return np.arccos(np.clip(F32(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]), -1, 1), dtype=np.float64)
+
+    tris = []
+    for yy in range(res):
+        for xx in range(res):
+            p1, p2, p3, p4 = x[at(xx, yy)], x[at(xx + 1, yy)], x[at(xx, yy + 1)], x[at(xx + 1, yy + 1)]
+            a1, a2, a3, a4 = angle(p3, p1, p2), angle(p1, p2, p4), angle(p1, p3, p4), angle(p3, p4, p2)
+            if a1 + a4 > a2 + a3:
+                tris += [(at(xx, yy), at(xx + 1, yy + 1), at(xx, yy + 1)),
+                         (at(xx, yy), at(xx + 1, yy), at(xx + 1, yy + 1))]
+            else:
+                tris += [(at(xx, yy), at(xx + 1, yy), at(xx, yy + 1)),
+                         (at(xx, yy + 1), at(xx + 1, yy), at(xx + 1, yy + 1))]
+    return x, np.asarray(tris, dtype=np.uint32)
+
+
+def mesh_cloth(x: np.ndarray, tris: np.ndarray, diameter: float, attached: Sequence[int] = ()) -> Cloth:
+    """A cloth from any triangle mesh: generated stretch / bending constraints (rest lengths and
+    rest angles of the given shape) and long-range attachments to ``attached`` vertices."""
+    x = np.ascontiguousarray(x, dtype=F32).reshape(-1, 3)
+    st, sr, bd, br = mesh_constraints(x, tris)
+    n = x.shape[0]
+    slots = np.zeros((len(attached), 3), dtype=F32)
+    pid, sid, dist = [], [], []
+    for s_, p in enumerate(attached):
+        slots[s_] = x[p]
+        pid.append(np.arange(n, dtype=np.int32))
+        sid.append(np.full(n, s_, dtype=np.int32))
+        dist.append(_length_f32(slots[s_][None, :] - x))
+    cat = lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t)
+    return Cloth(x=x, tris=np.ascontiguousarray(tris, dtype=np.uint32), stretch=st, stretch_rest=sr, bend=bd,
+                 bend_rest=br, slots=slots, attach_pid=cat(pid, np.int32), attach_slot=cat(sid, np.int32),
+                 attach_dist=cat(dist, F32), diameter=F32(diameter))
+
+
 @functools.lru_cache(maxsize=64)
 def grid_topology(n: int, bend_mode: str):
     """(triangles, stretch pairs, bends) of an n x n grid; shared read-only between cloths."""

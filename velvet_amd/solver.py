@@ -143,6 +143,13 @@ class ClothSolver:
         rest = np.ascontiguousarray(rest, dtype=F32).reshape(-1)
         check(lib().vtx_add_stretch_n(self.h, seg, _ip(ij), _fp(rest), rest.size))
 
+    def add_mesh_constraints(self, offset: int, tris, seg: int = 0):
+        """Stretch and bending constraints of the cloth added at ``offset``, generated from its
+        current positions (rest angles included)."""
+        t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
+        check(lib().vtx_add_mesh_constraints(self.h, seg, int(offset), t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                             t.size // 3))
+
     def AddAttachSlot(self, pos, seg: int = 0) -> int:
         p = np.ascontiguousarray(pos, dtype=F32).reshape(3)
         out = C.c_int32()
