@@ -202,6 +202,16 @@ int vtx_snapshot_size(vtx_solver* s, int32_t seg, uint64_t* bytes);
 int vtx_snapshot(vtx_solver* s, int32_t seg, void* buf, uint64_t bytes);
 int vtx_restore(vtx_solver* s, int32_t seg, const void* buf, uint64_t bytes);
 
+/* Extension replacing the reference's CUDA-GL interop (Velvet/VtBuffer.hpp:167-229) for a viewer
+ * that runs at its own pace: vtx_readback_request queues a copy of a segment's positions
+ * (float4, w = inverse mass) and normals (float4) into the older of two pinned host buffers,
+ * after the work queued so far; vtx_readback_latest returns the newest completed copy without
+ * waiting (*positions = NULL if none has completed) and the number of frames simulated before it
+ * was requested. A returned buffer stays valid until the request after next. */
+int vtx_readback_request(vtx_solver* s, int32_t seg);
+int vtx_readback_latest(vtx_solver* s, int32_t seg, const float** positions, const float** normals, uint32_t* n,
+                        uint64_t* frame);
+
 /* Host edit of one particle, the MouseGrabber effect (Velvet/MouseGrabber.hpp:51-52,60,74-75).
  * pos/vel may be NULL to keep them; inv_mass < 0 keeps it. */
 int vtx_set_particle(vtx_solver* s, int32_t seg, int32_t i, const float* pos, const float* vel,

@@ -434,3 +434,30 @@ def test_add_mesh_constraints_matches_registered_arrays():
             s.Simulate()
         s.synchronize()
         assert np.array_equal(s.positions(), xa) and np.array_equal(s.velocities(), va)
+
+
+def test_viewer_readback_double_buffer():
+    """Readbacks queued after frames return the positions and normals of exactly that frame, the
+    newest completed one first, with the frame count; nothing before the first request."""
+    say("viewer readback")
+    sc = S.config1(frames=4)
+    with ClothSolver() as s:
+        s.load_scene(sc)
+        assert s.readback_latest() is None
+        seen = []
+        for f in range(1, 4):
+            s.Simulate()
+            s.readback_request()
+            s.synchronize()
+            seen.append((s.positions(), s.normals()))
+            pos, nrm, frame = s.readback_latest()
+            assert frame == f
+            assert np.array_equal(pos[:, :3], seen[-1][0]) and np.array_equal(nrm[:, :3], seen[-1][1])
+            assert np.array_equal(pos[:, 3], s.inv_masses())
+        # two requests in a row: the newest wins; the older buffer kept the previous frame
+        s.Simulate()
+        s.readback_request()
+        s.readback_request()
+        s.synchronize()
+        pos, _, frame = s.readback_latest()
+        assert frame == 4 and np.array_equal(pos[:, :3], s.positions())

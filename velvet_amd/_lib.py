@@ -79,6 +79,8 @@ _SIGS = {
                                    P(u32), P(C.c_float), u32, P(u32)]),
     "vtx_add_mesh_constraints": (i32, [C.c_void_p, i32, i32, P(u32), u32]),
     "vtx_snapshot_size": (i32, [C.c_void_p, i32, P(C.c_uint64)]),
+    "vtx_readback_request": (i32, [C.c_void_p, i32]),
+    "vtx_readback_latest": (i32, [C.c_void_p, i32, P(P(C.c_float)), P(P(C.c_float)), P(u32), P(C.c_uint64)]),
     "vtx_snapshot": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),
     "vtx_restore": (i32, [C.c_void_p, i32, C.c_void_p, C.c_uint64]),
     "vtx_set_tuning": (i32, [C.c_void_p, P(VtxTuning)]),

@@ -318,6 +318,32 @@ int vtx_add_mesh_constraints(vtx_solver* s, int32_t seg, int32_t offset, const u
     API_BEGIN s->engine.add_mesh_constraints(seg, offset, tris, ntri); API_END
 }
 
+int vtx_readback_request(vtx_solver* s, int32_t seg) {
+    NEED(s);
+    API_BEGIN s->engine.readback_request(seg); API_END
+}
+
+int vtx_readback_latest(vtx_solver* s, int32_t seg, const float** positions, const float** normals, uint32_t* n,
+                        uint64_t* frame) {
+    NEED(s); NEED(positions); NEED(normals); NEED(n); NEED(frame);
+    API_BEGIN
+    const float4 *p = nullptr, *q = nullptr;
+    uint32_t m = 0;
+    uint64_t f = 0;
+    if (s->engine.readback_latest(seg, &p, &q, &m, &f)) {
+        *positions = reinterpret_cast<const float*>(p);
+        *normals = reinterpret_cast<const float*>(q);
+        *n = m;
+        *frame = f;
+    } else {
+        *positions = nullptr;
+        *normals = nullptr;
+        *n = 0;
+        *frame = 0;
+    }
+    API_END
+}
+
 int vtx_snapshot_size(vtx_solver* s, int32_t seg, uint64_t* bytes) {
     NEED(s); NEED(bytes);
     API_BEGIN *bytes = (uint64_t)s->engine.snapshot_bytes(seg); API_END

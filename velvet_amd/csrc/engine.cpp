@@ -418,6 +418,60 @@ void Engine::restore(int32_t s, const void* buf, size_t bytes) {
     }
 }
 
+Readback::~Readback() {
+    for (int k = 0; k < 2; k++) {
+        if (ev[k]) {
+            (void)hipEventSynchronize(ev[k]);
+            (void)hipEventDestroy(ev[k]);
+        }
+        if (host[k]) (void)hipHostFree(host[k]);
+    }
+}
+
+void Engine::readback_request(int32_t s) {
+    Segment& sg = seg(s);
+    ensure_committed();
+    VTX_REQUIRE(sg.active && sg.uploaded, VTX_ESTATE, "segment is not on the device");
+    const uint32_t n = sg.n();
+    if (!n) return;
+    if (!sg.rb || sg.rb->n != n) {
+        sg.rb.reset(new Readback);
+        for (int k = 0; k < 2; k++) {
+            VTX_HIP(hipHostMalloc((void**)&sg.rb->host[k], sizeof(float4) * 2 * (size_t)n, hipHostMallocDefault));
+            VTX_HIP(hipEventCreateWithFlags(&sg.rb->ev[k], hipEventDisableTiming));
+        }
+        sg.rb->n = n;
+    }
+    Readback& rb = *sg.rb;
+    const int k = rb.next;
+    VTX_HIP(hipMemcpyAsync(rb.host[k], pl_.X + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost, stream_));
+    VTX_HIP(hipMemcpyAsync(rb.host[k] + n, pl_.N + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost, stream_));
+    VTX_HIP(hipEventRecord(rb.ev[k], stream_));
+    rb.frame[k] = frames_;
+    rb.pending[k] = true;
+    rb.next = k ^ 1;
+}
+
+bool Engine::readback_latest(int32_t s, const float4** pos, const float4** nrm, uint32_t* n, uint64_t* frame) {
+    Segment& sg = seg(s);
+    if (!sg.rb) return false;
+    Readback& rb = *sg.rb;
+    for (int t = 0; t < 2; t++) {
+        const int k = rb.next ^ 1 ^ t;           // newest request first
+        if (!rb.pending[k]) continue;
+        const hipError_t e = hipEventQuery(rb.ev[k]);
+        if (e == hipSuccess) {
+            *pos = rb.host[k];
+            *nrm = rb.host[k] + rb.n;
+            *n = rb.n;
+            *frame = rb.frame[k];
+            return true;
+        }
+        if (e != hipErrorNotReady) VTX_HIP(e);
+    }
+    return false;
+}
+
 void Engine::invalidate_graph() {
     if (gexec_) {
         (void)hipGraphExecDestroy(gexec_);
@@ -1253,6 +1307,7 @@ void Engine::simulate(float frame_dt) {
     VTX_REQUIRE(frame_dt > 0.f, VTX_EINVAL, "frame_dt must be positive");
     ensure_committed();
     if (ds_.nparticles == 0) return;
+    frames_++;
     if (timing_) {
         stage_used_ = 0;
         record_frame(frame_dt, stream_);

@@ -71,6 +71,21 @@ struct Ranges {
 
 // One independent simulation: the state of one reference VtClothSolverGPU
 // (Velvet/VtClothSolverGPU.hpp:207-231).
+// Double-buffered device -> pinned-host copies of a segment's positions and normals for a
+// viewer: each request fills the older buffer after the work queued so far.
+struct Readback {
+    float4* host[2] = {nullptr, nullptr};    // n positions (w = inverse mass), then n normals
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    uint64_t frame[2] = {0, 0};
+    bool pending[2] = {false, false};
+    uint32_t n = 0;
+    int next = 0;
+    Readback() = default;
+    Readback(const Readback&) = delete;
+    Readback& operator=(const Readback&) = delete;
+    ~Readback();
+};
+
 struct Segment {
     vtx_params prm{};
     std::vector<float> x, v, w, x0;          // 3n, 3n, n, 3n (host copy; see `uploaded`)
@@ -90,6 +105,7 @@ struct Segment {
     Prepared prep;
     Ranges rg;
     SegDev dev{};                            // absolute bases filled at upload
+    std::unique_ptr<Readback> rb;            // viewer copies, created on first request
     uint32_t n() const { return (uint32_t)w.size(); }
     uint32_t table_size() const { return prm.hash_table_size > 0 ? (uint32_t)prm.hash_table_size : 2u * n(); }
     bool selfcol() const { return prm.enable_self_collision && n() > 0; }
@@ -172,6 +188,11 @@ public:
     float4* device_positions(int32_t s, uint32_t* n);
     float4* device_normals(int32_t s, uint32_t* n);
     void set_particle(int32_t s, int32_t i, const float* pos, const float* vel, float inv_mass);
+    // Queue a copy of segment s's positions and normals into the older of its two pinned
+    // buffers, after the work queued so far; the newest completed copy, without waiting
+    // (false if none has completed).
+    void readback_request(int32_t s);
+    bool readback_latest(int32_t s, const float4** pos, const float4** nrm, uint32_t* n, uint64_t* frame);
 
     void write_predicted(int32_t s, const float* xyz);
     void run_hash();
@@ -235,7 +256,7 @@ private:
     bool need_collide_ = false;
     int substeps_ = 1, iterations_ = 0, interleave_ = 1;
     uint64_t live_particles_ = 0;
-
+    uint64_t frames_ = 0;                    // frames simulated
     Tuning tun_;
     bool any_lra_dense_ = false;
     bool use_stencil_ = true;

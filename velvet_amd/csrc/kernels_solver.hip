@@ -189,16 +189,20 @@ __device__ __forceinline__ void stretch_term(V3 p, float w, float4 q, float rest
 }
 
 // One long-range attachment (Velvet/VtClothSolverGPU.cu:218-234).
+// Branch-free (selects): the slot position is used on every path, so its loads can be issued
+// ahead of the term instead of inside a conditional block.
 __device__ __forceinline__ void attach_term(V3 p, float w, float4 slot, float dist0, float lrs, V3& delta,
                                             int& cnt) {
     const float target = dist0 * lrs;
-    if (w == 0.f && target > 0.f) return;
     const V3 diff = p - xyz(slot);
     const float dist = fsqrt(dot(diff, diff));
-    if (dist > target) {
-        delta += -diff + diff * (target * frcp(dist));
-        cnt++;
-    }
+    // the reference returns early for a pinned particle with a positive target
+    const bool act = !(w == 0.f && target > 0.f) && dist > target;
+    const V3 c = -diff + diff * (target * frcp(dist));
+    delta.x = act ? delta.x + c.x : delta.x;
+    delta.y = act ? delta.y + c.y : delta.y;
+    delta.z = act ? delta.z + c.z : delta.z;
+    cnt += act ? 1 : 0;
 }
 
 // All attachments of one particle, in registration order: dense (slot-major distances, the slot
@@ -218,8 +222,16 @@ __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg
             float r[LB];
 #pragma unroll
             for (int u = 0; u < LB; u++) r[u] = lra[(size_t)(s + u) * n];
+            // slot positions (wave-uniform) four at a time, so that few are live at once
 #pragma unroll
-            for (int u = 0; u < LB; u++) attach_term(p, w, slot[s + u], r[u], sg.lrs, delta, cnt);
+            for (int g = 0; g < LB; g += 4) {
+                float4 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) q[u] = slot[s + g + u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) attach_term(p, w, q[u], r[g + u], sg.lrs, delta, cnt);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         for (; s < sg.scount; s++) attach_term(p, w, slot[s], lra[(size_t)s * n], sg.lrs, delta, cnt);
         return;

@@ -215,6 +215,21 @@ class ClothSolver:
         out["frame_ms"] = float(t.frame_ms)
         return out
 
+    def readback_request(self, seg: int = 0):
+        check(lib().vtx_readback_request(self.h, seg))
+
+    def readback_latest(self, seg: int = 0):
+        """(positions (n, 4), normals (n, 4), frame) of the newest completed readback, copied out
+        of the pinned buffer, or None."""
+        p, q = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+        n, f = C.c_uint32(), C.c_uint64()
+        check(lib().vtx_readback_latest(self.h, seg, C.byref(p), C.byref(q), C.byref(n), C.byref(f)))
+        if not p:
+            return None
+        pos = np.ctypeslib.as_array(p, shape=(n.value, 4)).copy()
+        nrm = np.ctypeslib.as_array(q, shape=(n.value, 4)).copy()
+        return pos, nrm, int(f.value)
+
     def snapshot(self, seg: int = 0) -> bytes:
         """Positions (+ inverse masses), velocities, initial and slot positions of a segment."""
         n = C.c_uint64()
