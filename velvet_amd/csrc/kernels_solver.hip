@@ -146,24 +146,44 @@ __global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float 
         V3 delta = v3(0.f, 0.f, 0.f);
         int cnt = 0;
         const V3 vel_i = p - x;
-        for (uint32_t k = 0; k < sg.max_nbr; k++) {
-            const uint32_t j = nb[(size_t)k * n];
-            if (j >= n) break;
-            const uint32_t jg = sg.pbase + j;
-            const float4 pj4 = P[jg];
-            const float denom = w + pj4.w;
-            if (denom <= 0.f) continue;
-            const V3 pj = xyz(pj4);
-            const V3 diff = p - pj;
-            const float dist = flen(diff);
-            if (dist >= sg.diameter) continue;
-            const V3 grad = diff * frcp(dist + VTX_EPSILON);
-            const float lambda = (dist - sg.diameter) * frcp(denom);
-            const V3 common = lambda * grad;
-            cnt++;
-            delta -= w * common;
-            const V3 relv = vel_i - (pj - xyz(d.X[jg]));
-            delta += w * friction_of(common, relv, sg.friction);
+        // neighbours four at a time: their indices in one round trip, their positions in the
+        // next, then the four terms in list order; an entry counts only if every entry before it
+        // is a particle index (the list ends at the first sentinel, later rows are stale)
+        bool live = true;
+        for (uint32_t k0 = 0; k0 < sg.max_nbr && live; k0 += 4) {
+            uint32_t jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) jj[u] = k0 + u < sg.max_nbr ? nb[(size_t)(k0 + u) * n] : VTX_INVALID;
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                live = live && jj[u] < n;
+                ok[u] = live;
+            }
+            float4 pj4[4], xj4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t jg = sg.pbase + (ok[u] ? jj[u] : 0u);
+                pj4[u] = P[jg];
+                xj4[u] = d.X[jg];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (!ok[u]) continue;
+                const float denom = w + pj4[u].w;
+                if (denom <= 0.f) continue;
+                const V3 pj = xyz(pj4[u]);
+                const V3 diff = p - pj;
+                const float dist = flen(diff);
+                if (dist >= sg.diameter) continue;
+                const V3 grad = diff * frcp(dist + VTX_EPSILON);
+                const float lambda = (dist - sg.diameter) * frcp(denom);
+                const V3 common = lambda * grad;
+                cnt++;
+                delta -= w * common;
+                const V3 relv = vel_i - (pj - xyz(xj4[u]));
+                delta += w * friction_of(common, relv, sg.friction);
+            }
         }
         if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     }
@@ -377,13 +397,18 @@ __device__ __forceinline__ int slot_index(int ra, int rb) {
     return TYPE == 2 ? (ra + 1) * 18 + (rb + 1) : (ra + 1) * 17 + (rb + 1);
 }
 
-// Evaluate bending slot j (index within its type's range) of a grid tile into LDS.
+// Evaluate the j-th bending slot of a grid tile's range for one type, taken in row-major or
+// (colmajor) column-major order, into LDS at its row-major index. A tile cut by the cloth's
+// right edge takes column-major order, so that whole 64-lane groups of absent slots are skipped.
 template <int TYPE>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
                                           float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int rows,
-                                          int cols, uint32_t j, float xpbd) {
+                                          int cols, uint32_t jj, float xpbd, bool colmajor) {
     constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
-    const int ra = (int)(j / W) - 1, rb = (int)(j % W) - 1;
+    constexpr int H = TYPE == 1 ? 18 : 17;                                  // owner rows of the range
+    const int ra = colmajor ? (int)(jj % H) - 1 : (int)(jj / W) - 1;
+    const int rb = colmajor ? (int)(jj / H) - 1 : (int)(jj % W) - 1;
+    const uint32_t j = (uint32_t)((ra + 1) * W + (rb + 1));                 // row-major storage index
     const int a = a0 + ra, b = b0 + rb;
     bool on = false;
     if (a >= 0 && a < rows && b >= 0 && b < cols) {
@@ -429,11 +454,12 @@ template <int TYPE>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
                                                float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
-                                               V3& delta, int& cnt) {
+                                               bool colmajor, V3& delta, int& cnt) {
     constexpr int NS = TYPE == 0 ? VTX_NSX : TYPE == 1 ? VTX_NSR : VTX_NSD;
     const uint32_t tid = threadIdx.x;
-    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
-    if (tid < NS - VTX_TILE) eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd);
+    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd, colmajor);
+    if (tid < NS - VTX_TILE)
+        eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd, colmajor);
     __syncthreads();
     if (inside) {
         // role r of the slot owned at (ty + dy_r, tx + dx_r), per type:
@@ -517,9 +543,10 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
-    bend_type_pass<0>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-    bend_type_pass<1>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-    bend_type_pass<2>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    const bool colmajor = t.order != 0;
+    bend_type_pass<0>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
+    bend_type_pass<1>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
+    bend_type_pass<2>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
     if (!inside) return;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);
