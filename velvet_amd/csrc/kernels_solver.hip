@@ -400,14 +400,14 @@ __device__ __forceinline__ int slot_index(int ra, int rb) {
 // Evaluate the j-th bending slot of a grid tile's range for one type, taken in row-major or
 // (colmajor) column-major order, into LDS at its row-major index. A tile cut by the cloth's
 // right edge takes column-major order, so that whole 64-lane groups of absent slots are skipped.
-template <int TYPE>
+template <int TYPE, bool COLMAJOR>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
                                           float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int rows,
-                                          int cols, uint32_t jj, float xpbd, bool colmajor) {
+                                          int cols, uint32_t jj, float xpbd) {
     constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
     constexpr int H = TYPE == 1 ? 18 : 17;                                  // owner rows of the range
-    const int ra = colmajor ? (int)(jj % H) - 1 : (int)(jj / W) - 1;
-    const int rb = colmajor ? (int)(jj / H) - 1 : (int)(jj % W) - 1;
+    const int ra = COLMAJOR ? (int)(jj % H) - 1 : (int)(jj / W) - 1;
+    const int rb = COLMAJOR ? (int)(jj / H) - 1 : (int)(jj % W) - 1;
     const uint32_t j = (uint32_t)((ra + 1) * W + (rb + 1));                 // row-major storage index
     const int a = a0 + ra, b = b0 + rb;
     bool on = false;
@@ -450,16 +450,16 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
 
 // One slot type: evaluate its slots (t, t + 256), then every tile vertex adds its four
 // memberships of this type in role order.
-template <int TYPE>
+template <int TYPE, bool COLMAJOR>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
                                                float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
-                                               bool colmajor, V3& delta, int& cnt) {
+                                               V3& delta, int& cnt) {
     constexpr int NS = TYPE == 0 ? VTX_NSX : TYPE == 1 ? VTX_NSR : VTX_NSD;
     const uint32_t tid = threadIdx.x;
-    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd, colmajor);
+    eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
     if (tid < NS - VTX_TILE)
-        eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd, colmajor);
+        eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd);
     __syncthreads();
     if (inside) {
         // role r of the slot owned at (ty + dy_r, tx + dx_r), per type:
@@ -543,10 +543,15 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
-    const bool colmajor = t.order != 0;
-    bend_type_pass<0>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
-    bend_type_pass<1>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
-    bend_type_pass<2>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, colmajor, delta, cnt);
+    if (t.order == 0) {   // tile-uniform: one branch, each order compiled on its own
+        bend_type_pass<0, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<1, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<2, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    } else {
+        bend_type_pass<0, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<1, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<2, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    }
     if (!inside) return;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);
