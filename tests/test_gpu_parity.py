@@ -461,3 +461,71 @@ def test_viewer_readback_double_buffer():
         s.synchronize()
         pos, _, frame = s.readback_latest()
         assert frame == 4 and np.array_equal(pos[:, :3], s.positions())
+
+
+def _lists_equal(nb, rnb):
+    """Column-major neighbour tables agree up to and including each list's first sentinel
+    (rows after it are not part of the list)."""
+    live_r = np.cumprod(rnb != SENT, axis=0).astype(bool)
+    live_g = np.cumprod(nb != SENT, axis=0).astype(bool)
+    assert np.array_equal(live_r, live_g)
+    assert np.array_equal(nb[live_r], rnb[live_r])
+
+
+def test_hash_stage_bit_exact_full_size():
+    """The whole hash stage at the 1024 x 1024 size (one segment, 2^21 cells), perturbed so the
+    lists are not the rest lattice: cell tables, sorted order and lists bit-exact."""
+    say("hash full size 1024^2")
+    sc = S.config3(frames=1)
+    c = sc.cloths[0]
+    sp = 2.0 / 1023.0
+    p = (c.x + S.uniform(5, 77, 3 * c.n, -0.3 * sp, 0.3 * sp).reshape(-1, 3)).astype(F32)
+    s = _hash_solver([(c, p)], [1 << 21])
+    cell = float(F32(F32(c.diameter) * F32(1.5)))
+    ph, rsh, rsi, rcs, rce, rnb = oracle.hash32(p, c.x, cell, float(c.diameter), 1 << 21)
+    cs, ce, sh, si, nb = s.read_hash(0)
+    s.close()
+    np.testing.assert_array_equal(cs, rcs)
+    valid = rcs != SENT
+    np.testing.assert_array_equal(ce[valid], rce[valid])
+    np.testing.assert_array_equal(sh, rsh)
+    np.testing.assert_array_equal(si, rsi)
+    _lists_equal(nb, rnb)
+    say(f"  {int(valid.sum())} occupied cells, {int((rnb != SENT).sum())} list entries")
+
+
+def test_empty_and_ragged_segments():
+    """An empty solver steps; a batch mixing an empty segment, a one-particle segment, a cloth
+    with no constraints, an inactive segment and cloths of different sizes gives each active
+    segment its solo result, and leaves the inactive one where it was."""
+    say("empty and ragged segments")
+    with ClothSolver() as s:
+        s.Simulate()
+        s.synchronize()
+        assert s.num_particles(0) == 0 and s.positions(0).shape == (0, 3)
+    from tests.test_oracle_pins import tiny_scene
+    # active segments share the substep loop: same substeps x iterations (and hash interleave)
+    one = tiny_scene([[0.1, 1.0, 0.2]], gravity=(0.0, -9.8, 0.0), enable_self_collision=True, num_substeps=5,
+                     num_iterations=10)
+    base = S.config4(count=3, frames=3, smax=30)
+    c = base[0].cloths[0]
+    bare_c = S.Cloth(**{**c.__dict__, "stretch": np.zeros((0, 2), np.int32), "stretch_rest": np.zeros(0, F32),
+                        "bend": np.zeros((0, 4), np.uint32), "bend_rest": np.zeros(0, F32),
+                        "slots": np.zeros((0, 3), F32), "attach_pid": np.zeros(0, np.int32),
+                        "attach_slot": np.zeros(0, np.int32), "attach_dist": np.zeros(0, F32)})
+    bare = S.Scene(base[0].params, [bare_c], base[0].colliders, 3)
+    scenes = [base[1], one, bare, base[2], base[0]]
+    alone = [_run_gpu(sc, 3, use_graph=False)[:2] for sc in scenes[:4]]
+    with ClothSolver() as s:
+        segs = [s.load_scene(sc) for sc in scenes]
+        empty = s.add_segment(base[0].params)
+        s.set_segment_active(segs[4], False)
+        x_parked = s.positions(segs[4])
+        for _ in range(3):
+            s.Simulate()
+        s.synchronize()
+        for k in range(4):
+            assert np.array_equal(s.positions(segs[k]), alone[k][0]), k
+            assert np.array_equal(s.velocities(segs[k]), alone[k][1]), k
+        assert s.num_particles(empty) == 0
+        assert np.array_equal(s.positions(segs[4]), x_parked)
