@@ -152,7 +152,8 @@ typedef struct vtx_tuning {
     int32_t lra_batch;         /* dense attachment distances loaded per batch: 4, 16 or 64 */
     int32_t xcd_remap;         /* hand each XCD a contiguous run of tiles */
     float frame_ms;            /* vtx_autotune: the chosen variant's frame time (0 from the cache) */
-    int32_t pad;
+    int32_t gather_block;      /* threads per workgroup of the neighbour search and contact pass:
+                                  64, 128 or 256 (each 256-particle tile split across workgroups) */
 } vtx_tuning;
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);
 int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);

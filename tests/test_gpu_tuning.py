@@ -1,14 +1,14 @@
 """Launch variants and the tuner on the GPU through the C ABI.
 
-* every combination of the tuning knobs (graph replay, neighbour-search thread order, dense
-  attachment batch, XCD remap) gives bit-identical positions and velocities, on a batch that has
+* every value of the tuning knobs (graph replay, neighbour-search thread order, dense attachment
+  batch, XCD remap, gather-kernel block size), alone and mixed, gives bit-identical positions and
+  velocities, on a batch that has
   grid cloths (patch order) and on one whose segments are general meshes (two cloths sharing a
   segment);
 * vtx_autotune leaves the simulation state where it was (trials are undone) and a later run
   under the tuned variant equals an untuned run bit for bit;
 * a second solver of the same shape class takes the cached tuning without timing.
 """
-import itertools
 import sys
 
 import numpy as np
@@ -49,15 +49,24 @@ def _run(scenes, frames, **tuning):
         return [(s.positions(k), s.velocities(k)) for k in range(len(scenes))]
 
 
+# every value of every knob on its own, plus mixed combinations
+VARIANTS = ([dict(use_graph=g) for g in (0, 1)] + [dict(neighbor_order=o) for o in (0, 1, 2)]
+            + [dict(lra_batch=b) for b in (4, 16, 64)] + [dict(xcd_remap=r) for r in (0, 1)]
+            + [dict(gather_block=b) for b in (64, 128, 256)]
+            + [dict(use_graph=1, neighbor_order=2, lra_batch=4, xcd_remap=0, gather_block=64),
+               dict(use_graph=0, neighbor_order=0, lra_batch=64, xcd_remap=1, gather_block=128),
+               dict(use_graph=1, neighbor_order=1, lra_batch=16, xcd_remap=0, gather_block=64)])
+
+
 @pytest.mark.parametrize("which", ["grid batch", "general"])
 def test_variants_bit_identical(which):
     scenes = _batch() if which == "grid batch" else _general()
     ref = _run(scenes, 4)
-    for g, so, lb, rm in itertools.product((0, 1), (0, 1, 2), (4, 16, 64), (0, 1)):
-        say(f"{which}: graph={g} neighbor_order={so} lra_batch={lb} remap={rm}")
-        got = _run(scenes, 4, use_graph=g, neighbor_order=so, lra_batch=lb, xcd_remap=rm)
-        for k, ((x, v), (xr, vr)) in enumerate(zip(got, ref)):
-            assert np.array_equal(x, xr) and np.array_equal(v, vr), (k, g, so, lb, rm)
+    for v in VARIANTS:
+        say(f"{which}: {v}")
+        got = _run(scenes, 4, **v)
+        for k, ((x, vel), (xr, vr)) in enumerate(zip(got, ref)):
+            assert np.array_equal(x, xr) and np.array_equal(vel, vr), (k, v)
 
 
 def test_variants_bit_identical_long_range_attachments():
@@ -101,3 +110,5 @@ def test_tuning_rejects_bad_values():
             s.set_tuning(lra_batch=7)
         with pytest.raises(Exception):
             s.set_tuning(neighbor_order=3)
+        with pytest.raises(Exception):
+            s.set_tuning(gather_block=96)

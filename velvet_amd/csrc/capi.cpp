@@ -210,7 +210,7 @@ static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
     out->lra_batch = t.lra_batch;
     out->xcd_remap = t.remap ? 1 : 0;
     out->frame_ms = (float)ms;
-    out->pad = 0;
+    out->gather_block = t.gather_block;
 }
 
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
@@ -226,6 +226,7 @@ int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
     k.nbr_order = t->neighbor_order;
     k.lra_batch = t->lra_batch;
     k.remap = t->xcd_remap != 0;
+    k.gather_block = t->gather_block;
     s->engine.set_tuning(k);
     API_END
 }

@@ -1287,7 +1287,7 @@ void Engine::hash_build(int in, hipStream_t s) {
     VTX_HIP(launch_cell_bounds(ds_, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, in, tun_.nbr_order, s));
+    VTX_HIP(launch_neighbors(ds_, in, tun_.nbr_order, tun_.gather_block, s));
     stage_end(s);
     sorted_src_ = src;
 }
@@ -1304,7 +1304,7 @@ void Engine::record_frame(float frame_dt, hipStream_t s) {
         if (any_selfcol_ && sub % interleave_ == 0) hash_build(cur, s);
         if (need_collide_) {
             stage_begin("k_collide", s);
-            VTX_HIP(launch_collide(ds_, cur, h, s));
+            VTX_HIP(launch_collide(ds_, cur, h, tun_.gather_block, s));
             stage_end(s);
             cur ^= 1;
         }
@@ -1381,6 +1381,8 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.lra_batch == 4 || t.lra_batch == 16 || t.lra_batch == 64, VTX_EINVAL,
                 "lra_batch must be 4, 16 or 64");
     VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
+    VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
+                "gather_block must be 64, 128 or 256");
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
     invalidate_graph();
@@ -1468,10 +1470,11 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         };
         bms = trial(best);
         // one knob at a time, every other value of it against the best so far
-        for (int knob = 0; knob < 4; knob++) {
+        for (int knob = 0; knob < 5; knob++) {
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
-            const int nval = knob == 1 || knob == 2 ? 3 : 2;
+            if (knob == 4 && !need_collide_) continue;
+            const int nval = knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
                 Tuning t = base;
@@ -1479,8 +1482,10 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
                 if (knob == 1) t.nbr_order = v;
                 if (knob == 2) t.lra_batch = v == 0 ? 4 : v == 1 ? 16 : 64;
                 if (knob == 3) t.remap = v != 0;
+                if (knob == 4) t.gather_block = 64 << v;
                 if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
-                                : knob == 2 ? t.lra_batch == base.lra_batch : t.remap == base.remap)
+                                : knob == 2 ? t.lra_batch == base.lra_batch
+                                : knob == 3 ? t.remap == base.remap : t.gather_block == base.gather_block)
                     continue;
                 const double ms = trial(t);
                 if (ms < 0.99 * bms) {

@@ -119,6 +119,7 @@ struct Tuning {
     int nbr_order = 2;        // neighbour search thread order: 0 particle, 1 sorted, 2 grid patches
     int lra_batch = 16;       // dense attachment distances loaded per batch: 4, 16 or 64
     bool remap = true;        // XCD-aware workgroup -> tile remap
+    int gather_block = 256;   // threads per workgroup of the neighbour search and contact pass
 };
 
 struct StageTime {

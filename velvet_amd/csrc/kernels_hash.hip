@@ -263,23 +263,24 @@ __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (size_t)((i & 0x1FFFFFFu) << 4));
 }
 
-template <int ORDER>
-__global__ __launch_bounds__(VTX_TILE) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
+template <int ORDER, int BS>
+__global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
 #pragma clang fp contract(off)
-    __shared__ uint32_t crange[VTX_NB_CACHE][VTX_TILE];   // lane-private columns: no barrier needed
-    uint32_t seg, i;
+    __shared__ uint32_t crange[VTX_NB_CACHE][BS];   // thread-private columns: no barrier needed
+    uint32_t seg, i, tix, lane;
+    tile_lane<VTX_TILE, BS>(d.remap, tix, lane);
     if (ORDER == NBR_PATCH) {
-        const GridTileDev t = d.gtiles[tile_of(d.remap)];
+        const GridTileDev t = d.gtiles[tix];
         seg = t.seg;
         const SegDev& s0 = d.segs[seg];
-        const uint32_t a = t.a0 + threadIdx.x / VTX_GT, b = t.b0 + threadIdx.x % VTX_GT;
+        const uint32_t a = t.a0 + lane / VTX_GT, b = t.b0 + lane % VTX_GT;
         if (a >= s0.rows || b >= s0.cols) return;
         i = a * s0.cols + b;
     } else {
-        const TileDev t = tiles[tile_of(d.remap)];
-        if (threadIdx.x >= t.count) return;
+        const TileDev t = tiles[tix];
+        if (lane >= t.count) return;
         seg = t.seg;
-        i = t.begin + threadIdx.x - d.segs[seg].pbase;         // sorted position or particle
+        i = t.begin + lane - d.segs[seg].pbase;                 // sorted position or particle
     }
     const SegDev sg = d.segs[seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
@@ -410,16 +411,26 @@ hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const DevState& d, int in, int order, hipStream_t s) {
-    const dim3 bl(VTX_TILE);
+template <int BS>
+static void launch_neighbors_bs(const DevState& d, int in, int order, hipStream_t s) {
+    constexpr uint32_t S = VTX_TILE / BS;
+    const dim3 bl(BS);
     if (order == NBR_PARTICLE) {
-        hipLaunchKernelGGL(k_neighbors<NBR_PARTICLE>, dim3(d.ntiles), bl, 0, s, d, d.tiles, in);
+        hipLaunchKernelGGL((k_neighbors<NBR_PARTICLE, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, in);
     } else if (order == NBR_SORTED) {
-        hipLaunchKernelGGL(k_neighbors<NBR_SORTED>, dim3(d.ntiles), bl, 0, s, d, d.tiles, in);
+        hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, in);
     } else {   // grid segments by patches, the others in sorted order
-        if (d.ngtiles) hipLaunchKernelGGL(k_neighbors<NBR_PATCH>, dim3(d.ngtiles), bl, 0, s, d, d.itiles, in);
-        if (d.nitiles) hipLaunchKernelGGL(k_neighbors<NBR_SORTED>, dim3(d.nitiles), bl, 0, s, d, d.itiles, in);
+        if (d.ngtiles)
+            hipLaunchKernelGGL((k_neighbors<NBR_PATCH, BS>), dim3(d.ngtiles * S), bl, 0, s, d, d.itiles, in);
+        if (d.nitiles)
+            hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.nitiles * S), bl, 0, s, d, d.itiles, in);
     }
+}
+
+hipError_t launch_neighbors(const DevState& d, int in, int order, int block, hipStream_t s) {
+    if (block == 64) launch_neighbors_bs<64>(d, in, order, s);
+    else if (block == 128) launch_neighbors_bs<128>(d, in, order, s);
+    else launch_neighbors_bs<256>(d, in, order, s);
     return hipGetLastError();
 }
 

@@ -148,11 +148,14 @@ __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float fram
 
 // CollideParticles_Kernel + ApplyDeltas (Velvet/VtClothSolverGPU.cu:329-386) followed by the
 // substep's CollideSDF (cu:289-314, hpp:87).
-__global__ __launch_bounds__(VTX_TILE) void k_collide(DevState d, int in, float h) {
-    const TileDev t = d.tiles[tile_of(d.remap)];
-    if (threadIdx.x >= t.count) return;
+template <int BS>
+__global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
+    uint32_t tix, lane;
+    tile_lane<VTX_TILE, BS>(d.remap, tix, lane);
+    const TileDev t = d.tiles[tix];
+    if (lane >= t.count) return;
     const SegDev sg = d.segs[t.seg];
-    const uint32_t g = t.begin + threadIdx.x;
+    const uint32_t g = t.begin + lane;
     const float4* __restrict__ P = d.P[in];
     const float4 p4 = P[g];
     V3 p = xyz(p4);
@@ -624,9 +627,11 @@ hipError_t launch_frame_begin(const DevState& d, float frame_dt, float h, hipStr
     return last_launch();
 }
 
-hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s) {
+hipError_t launch_collide(const DevState& d, int in, float h, int block, hipStream_t s) {
     if (!d.ntiles) return hipSuccess;
-    hipLaunchKernelGGL(k_collide, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, h);
+    if (block == 64) hipLaunchKernelGGL(k_collide<64>, dim3(d.ntiles * 4), dim3(64), 0, s, d, in, h);
+    else if (block == 128) hipLaunchKernelGGL(k_collide<128>, dim3(d.ntiles * 2), dim3(128), 0, s, d, in, h);
+    else hipLaunchKernelGGL(k_collide<256>, dim3(d.ntiles), dim3(256), 0, s, d, in, h);
     return last_launch();
 }
 

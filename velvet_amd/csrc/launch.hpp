@@ -9,7 +9,8 @@ namespace vtx {
 enum { EPILOGUE_NONE = 0, EPILOGUE_FINALIZE = 1, EPILOGUE_FINALIZE_PREDICT = 2 };
 
 hipError_t launch_frame_begin(const DevState& d, float frame_dt, float h, hipStream_t s);
-hipError_t launch_collide(const DevState& d, int in, float h, hipStream_t s);
+// block: threads per workgroup of the gather kernels (64, 128 or 256; same results)
+hipError_t launch_collide(const DevState& d, int in, float h, int block, hipStream_t s);
 // lra_batch: dense attachment distances loaded per batch (4, 16 or 64; same results)
 hipError_t launch_iterate(const DevState& d, int in, float h, int epilogue, bool constraints, int lra_batch,
                           hipStream_t s);
@@ -24,6 +25,6 @@ hipError_t launch_clear_cells(const DevState& d, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
 // Thread -> particle order of the neighbour search (same tables for every order).
 enum { NBR_PARTICLE = 0, NBR_SORTED = 1, NBR_PATCH = 2 };
-hipError_t launch_neighbors(const DevState& d, int in, int order, hipStream_t s);
+hipError_t launch_neighbors(const DevState& d, int in, int order, int block, hipStream_t s);
 
 }  // namespace vtx

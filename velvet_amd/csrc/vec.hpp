@@ -55,4 +55,15 @@ __device__ __forceinline__ uint32_t tile_of(uint32_t remap) {
     return remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
 }
 
+// A launch whose TILE-particle tiles are each covered by TILE / BS workgroups of BS threads:
+// the tile of the current workgroup and the current thread's lane within it (the sub-blocks of
+// one tile stay consecutive, so the XCD remap keeps them on one XCD).
+template <int TILE, int BS>
+__device__ __forceinline__ void tile_lane(uint32_t remap, uint32_t& tile, uint32_t& lane) {
+    constexpr uint32_t S = TILE / BS;
+    const uint32_t b = tile_of(remap);
+    tile = b / S;
+    lane = (b % S) * BS + threadIdx.x;
+}
+
 }  // namespace vtx
