@@ -85,13 +85,21 @@ def test_autotune_keeps_state_and_caches():
     with ClothSolver() as s:
         for sc in scenes:
             s.load_scene(sc)
+        s.Simulate()
+        s.synchronize()
         x0 = [s.positions(k) for k in range(len(scenes))]
+        n0 = [s.normals(k) for k in range(len(scenes))]
+        s.readback_request(0)
+        s.synchronize()
         t = s.autotune(frames=2)
         say(f"  tuned {t}")
-        assert t["frame_ms"] > 0 and t["lra_batch"] in (4, 16)
+        assert t["frame_ms"] > 0 and t["lra_batch"] in (4, 16, 64)
         for k in range(len(scenes)):
-            assert np.array_equal(s.positions(k), x0[k])
-        for _ in range(3):
+            assert np.array_equal(s.positions(k), x0[k]) and np.array_equal(s.normals(k), n0[k])
+        s.readback_request(0)
+        s.synchronize()
+        assert s.readback_latest(0)[2] == 1          # tuning frames are not counted
+        for _ in range(2):
             s.Simulate()
         s.synchronize()
         for k in range(len(scenes)):

@@ -1430,30 +1430,37 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         }
     }
     const size_t n = ap_.cap;
-    float4 *sx = nullptr, *sv = nullptr;
+    float4 *sx = nullptr, *sv = nullptr, *sn = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timing = timing_;
+    const uint64_t frames_before = frames_;
     timing_ = false;
     auto cleanup = [&] {
         (void)hipStreamSynchronize(stream_);
         if (sx) (void)hipFree(sx);
         if (sv) (void)hipFree(sv);
+        if (sn) (void)hipFree(sn);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         timing_ = timing;
+        frames_ = frames_before;
     };
     Tuning best = tun_;
     double bms = 0.0;
     try {
         VTX_HIP(hipMalloc((void**)&sx, n * sizeof(float4)));
         VTX_HIP(hipMalloc((void**)&sv, n * sizeof(float4)));
+        VTX_HIP(hipMalloc((void**)&sn, n * sizeof(float4)));
         VTX_HIP(hipEventCreate(&e0));
         VTX_HIP(hipEventCreate(&e1));
+        // positions and velocities are the state a frame starts from; normals are its output
         VTX_HIP(hipMemcpyAsync(sx, pl_.X, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         VTX_HIP(hipMemcpyAsync(sv, pl_.V, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        VTX_HIP(hipMemcpyAsync(sn, pl_.N, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         auto restore = [&] {
             VTX_HIP(hipMemcpyAsync(pl_.X, sx, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             VTX_HIP(hipMemcpyAsync(pl_.V, sv, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+            VTX_HIP(hipMemcpyAsync(pl_.N, sn, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         };
         auto trial = [&](const Tuning& t) {
             set_tuning(t);
