@@ -529,3 +529,15 @@ def test_empty_and_ragged_segments():
             assert np.array_equal(s.velocities(segs[k]), alone[k][1]), k
         assert s.num_particles(empty) == 0
         assert np.array_equal(s.positions(segs[4]), x_parked)
+
+
+def test_config3_full_size_vs_fp64():
+    """The 1024 x 1024 cloth (1,048,576 particles, self-collision, 3 spheres, 4 x 8) for two frames
+    against the FP64 oracle (the oracle takes ~20 s)."""
+    say("config3 full size 2 frames vs fp64")
+    sc = S.config3(frames=2)
+    x, _, _ = _run_gpu(sc, 2, expect_stencil=1)
+    ref = oracle.simulate(sc, frames=2).x.reshape(-1, 3)
+    err = float(np.max(np.abs(x.astype(np.float64) - ref)))
+    say(f"  max err {err:.3e} (extent {sc.extent():.3f})")
+    assert err <= 1e-3 * sc.extent()
