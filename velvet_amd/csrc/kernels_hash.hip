@@ -310,42 +310,37 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
         nb[0] = VTX_INVALID;
         return;
     }
-    // position of the next candidate (called only while candidates remain)
+    // position of the next candidate (called only while candidates remain); moving on to the
+    // next non-empty cell is branch-free unless that cell lies beyond the LDS cache
     uint32_t z = 0, s = 0, left = 0;
     auto next = [&]() -> uint32_t {
-        if (left == 0) {
+        const bool sw = left == 0;
+        const uint32_t e = crange[z < VTX_NB_CACHE ? z : VTX_NB_CACHE - 1u][threadIdx.x];
+        uint32_t st = e & 0x1FFFFFFu, ln = e >> 25;
+        if (sw && z >= VTX_NB_CACHE) {
             const int q = __builtin_ctz(mask);
-            mask &= mask - 1;
-            uint32_t st, ln;
-            if (z < VTX_NB_CACHE) {
-                const uint32_t e = crange[z][threadIdx.x];
-                st = e & 0x1FFFFFFu;
-                ln = e >> 25;
-            } else {
-                const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
-                st = c.x;
-                ln = min(c.y, c.x + K) - c.x;
-            }
-            z++;
-            s = st;
-            left = ln;
+            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
+            st = c.x;
+            ln = min(c.y, c.x + K) - c.x;
         }
-        left--;
+        mask = sw ? (mask & (mask - 1u)) : mask;
+        z += sw ? 1u : 0u;
+        s = sw ? st : s;
+        left = (sw ? ln : left) - 1u;
         return s++;
     };
-    // candidate test in walk order; true once the list is full
+    // candidate test in walk order, one predicate and one predicated store; a full list takes
+    // no more entries
     uint32_t k = 0;
-    auto take = [&](float4 qp, float4 r) -> bool {
+    auto take = [&](float4 qp, float4 r) {
         const uint32_t j = __float_as_uint(qp.w);
-        if (j == id) return false;                           // the particle itself
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        if (!(d2 < sg.cell2)) return false;
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-        if (!(e2 > sg.diam2)) return false;
-        nb[(size_t)k * n] = j;
-        return ++k >= K;
+        const bool acc = (j != id) & (d2 < sg.cell2) & (e2 > sg.diam2) & (k < K);
+        if (acc) nb[(size_t)k * n] = j;
+        k += acc ? 1u : 0u;
     };
     // two candidates per trip with alternating load buffers: the loads of the next candidate
     // are in flight while the current one is tested
@@ -358,16 +353,16 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
             qb = ld_sorted(sP, cur);
             rb = ld_sorted(sX0, cur);
         }
-        if (take(qa, ra)) return;                            // full: no sentinel
+        take(qa, ra);
         if (!two) break;
         if (it + 2 < total) {
             cur = next();
             qa = ld_sorted(sP, cur);
             ra = ld_sorted(sX0, cur);
         }
-        if (take(qb, rb)) return;
+        take(qb, rb);
     }
-    nb[(size_t)k * n] = VTX_INVALID;
+    if (k < K) nb[(size_t)k * n] = VTX_INVALID;                  // a full list has no sentinel
 }
 
 // ---------------------------------------------------------------- host launchers
