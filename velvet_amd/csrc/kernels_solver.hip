@@ -168,27 +168,31 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
         V3 delta = v3(0.f, 0.f, 0.f);
         int cnt = 0;
         const V3 vel_i = p - x;
-        // neighbours four at a time: their indices in one round trip, their positions in the
-        // next, then the four terms in list order; an entry counts only if every entry before it
-        // is a particle index (the list ends at the first sentinel, later rows are stale)
+        // neighbours four at a time: while a group's positions load, the next group's indices
+        // load too; the four terms are then added in list order. An entry counts only if every
+        // entry before it is a particle index (the list ends at the first sentinel, later rows are
+        // stale).
         bool live = true;
-        for (uint32_t k0 = 0; k0 < sg.max_nbr && live; k0 += 4) {
-            uint32_t jj[4];
+        uint32_t jn[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) jj[u] = k0 + u < sg.max_nbr ? nb[(size_t)(k0 + u) * n] : VTX_INVALID;
+        for (int u = 0; u < 4; u++) jn[u] = (uint32_t)u < sg.max_nbr ? nb[(size_t)u * n] : VTX_INVALID;
+        for (uint32_t k0 = 0; k0 < sg.max_nbr && live; k0 += 4) {
             bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                live = live && jj[u] < n;
+                live = live && jn[u] < n;
                 ok[u] = live;
             }
             float4 pj4[4], xj4[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint32_t jg = sg.pbase + (ok[u] ? jj[u] : 0u);
+                const uint32_t jg = sg.pbase + (ok[u] ? jn[u] : 0u);
                 pj4[u] = P[jg];
                 xj4[u] = d.X[jg];
             }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                jn[u] = live && k0 + 4 + u < sg.max_nbr ? nb[(size_t)(k0 + 4 + u) * n] : VTX_INVALID;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (!ok[u]) continue;
