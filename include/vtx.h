@@ -154,8 +154,11 @@ typedef struct vtx_tuning {
     float frame_ms;            /* vtx_autotune: the chosen variant's frame time (0 from the cache) */
     int32_t gather_block;      /* threads per workgroup of the neighbour search and contact pass:
                                   64, 128 or 256 (each 256-particle tile split across workgroups) */
+    int32_t streams;           /* 2: the segments split into two particle-balanced groups whose
+                                  frames run concurrently on two streams; 1: one stream */
+    int32_t pad;
 } vtx_tuning;
-int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);
+int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);   /* vtx_set_tuning ignores frame_ms and pad */
 int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);
 /* Time the variants on the current live set (coordinate descent, `frames` timed frames each,
  * positions and velocities restored after every trial, so the simulation does not advance),

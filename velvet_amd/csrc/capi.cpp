@@ -211,6 +211,8 @@ static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
     out->xcd_remap = t.remap ? 1 : 0;
     out->frame_ms = (float)ms;
     out->gather_block = t.gather_block;
+    out->streams = t.streams;
+    out->pad = 0;
 }
 
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
@@ -227,6 +229,7 @@ int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
     k.lra_batch = t->lra_batch;
     k.remap = t->xcd_remap != 0;
     k.gather_block = t->gather_block;
+    k.streams = t->streams;
     s->engine.set_tuning(k);
     API_END
 }

@@ -272,6 +272,12 @@ Engine::~Engine() {
     if (hist_) (void)hipFree(hist_);
     if (part_) (void)hipFree(part_);
     if (flags_) (void)hipFree(flags_);
+    if (stream2_) {
+        (void)hipStreamSynchronize(stream2_);
+        (void)hipStreamDestroy(stream2_);
+    }
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1091,6 +1097,20 @@ void Engine::rebuild_tables() {
     live_particles_ = 0;
     uint32_t maxbits = 0, hbase = 0, compact = 0;
     bool first = true;
+    // two groups (two streams): the first segments up to half of the particles, then the rest;
+    // each group sorts on its own, so its scan positions restart at zero
+    uint64_t total_n = 0;
+    uint32_t nlive = 0;
+    for (auto& sp : segs_)
+        if (sp && sp->active && sp->uploaded && sp->n()) {
+            total_n += sp->n();
+            nlive++;
+        }
+    const bool split = tun_.streams == 2 && nlive >= 2 && !timing_;
+    uint64_t cum = 0;
+    bool in_second = false;
+    size_t t_split = 0, i_split = 0, g_split = 0, s_split = 0;
+    uint32_t h_split = 0, p_split = 0;
     for (size_t si = 0; si < segs_.size(); si++) {
         Segment* sgp = segs_[si].get();
         if (!sgp || !sgp->active || !sgp->uploaded || sgp->n() == 0) continue;
@@ -1104,6 +1124,17 @@ void Engine::rebuild_tables() {
         }
         VTX_REQUIRE(p.num_substeps == substeps_ && p.num_iterations == iterations_ && p.interleaved_hash == interleave_,
                     VTX_EINVAL, "active segments of one solver must share substeps, iterations and interleave");
+        if (split && !in_second && cum > 0 && 2 * cum >= total_n) {
+            in_second = true;
+            t_split = htile.size();
+            i_split = hitile.size();
+            g_split = hgtile.size();
+            s_split = hst.size();
+            h_split = hbase;
+            p_split = (uint32_t)cum;
+            compact = 0;
+        }
+        cum += sg.n();
         const uint32_t ti = (uint32_t)hseg.size();
         table_seg_.push_back((int32_t)si);
         hseg.push_back(sg.dev);
@@ -1147,7 +1178,7 @@ void Engine::rebuild_tables() {
     }
     passes_ = std::max(1, (int)((maxbits + 7) / 8));
     VTX_REQUIRE((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK <= 65536u, VTX_ECAPACITY, "histogram too large");
-    const size_t npart = (hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1;
+    const size_t npart = 2 * ((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1);   // room for two groups
     if (hbase > hist_cap_) {
         if (hist_) (void)hipFree(hist_);
         hist_cap_ = hbase + hbase / 2;
@@ -1165,7 +1196,7 @@ void Engine::rebuild_tables() {
     }
 
     DevState d{};
-    d.nparticles = compact;
+    d.nparticles = (uint32_t)total_n;
     d.ntiles = (uint32_t)htile.size();
     d.nitiles = (uint32_t)hitile.size();
     d.ngtiles = (uint32_t)hgtile.size();
@@ -1207,7 +1238,37 @@ void Engine::rebuild_tables() {
     d.cols = pl_.cols;
     d.hist = hist_;
     d.part = part_;
+    d.hbase0 = 0;
     ds_ = d;
+    groups_.clear();
+    if (in_second) {
+        DevState g0 = d, g1 = d;
+        g0.ntiles = (uint32_t)t_split;
+        g0.nitiles = (uint32_t)i_split;
+        g0.ngtiles = (uint32_t)g_split;
+        g0.nstiles = (uint32_t)s_split;
+        g0.nhist = h_split;
+        g0.nparticles = p_split;
+        g1.tiles = d.tiles + t_split;
+        g1.itiles = d.itiles + i_split;
+        g1.gtiles = d.gtiles + g_split;
+        g1.stiles = d.stiles + s_split;
+        g1.ntiles = d.ntiles - (uint32_t)t_split;
+        g1.nitiles = d.nitiles - (uint32_t)i_split;
+        g1.ngtiles = d.ngtiles - (uint32_t)g_split;
+        g1.nstiles = d.nstiles - (uint32_t)s_split;
+        g1.hbase0 = h_split;
+        g1.nhist = d.nhist - h_split;
+        g1.nparticles = d.nparticles - p_split;
+        g1.part = part_ + part_cap_ / 2;
+        groups_.push_back(g0);
+        groups_.push_back(g1);
+        if (!stream2_) VTX_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+        if (!ev_fork_) VTX_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+        if (!ev_join_) VTX_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    } else {
+        groups_.push_back(d);
+    }
     tables_dirty_ = false;
 }
 
@@ -1271,40 +1332,40 @@ void Engine::stage_end(hipStream_t s) {
 }
 
 // SpatialHashGPU::Hash -> HashObjects (Velvet/SpatialHashGPU.hpp:41-52, Velvet/SpatialHashGPU.cu:159-196).
-void Engine::hash_build(int in, hipStream_t s) {
+void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
     stage_begin("k_hash_keys", s);
-    VTX_HIP(launch_hash_keys(ds_, in, s));
+    VTX_HIP(launch_hash_keys(d, in, s));
     stage_end(s);
     stage_begin("hash_sort", s);
     int src = 0;
     for (int pass = 0; pass < passes_; pass++) {
-        VTX_HIP(launch_sort_pass(ds_, src, 8 * pass, s));
+        VTX_HIP(launch_sort_pass(d, src, 8 * pass, s));
         src ^= 1;
     }
     stage_end(s);
     stage_begin("hash_cells", s);
-    VTX_HIP(launch_clear_cells(ds_, s));
-    VTX_HIP(launch_cell_bounds(ds_, in, src, s));
+    VTX_HIP(launch_clear_cells(d, s));
+    VTX_HIP(launch_cell_bounds(d, in, src, s));
     stage_end(s);
     stage_begin("k_neighbors", s);
-    VTX_HIP(launch_neighbors(ds_, in, tun_.nbr_order, tun_.gather_block, s));
+    VTX_HIP(launch_neighbors(d, in, tun_.nbr_order, tun_.gather_block, s));
     stage_end(s);
     sorted_src_ = src;
 }
 
-// VtClothSolverGPU::Simulate stage order (Velvet/VtClothSolverGPU.hpp:56-111). In timing mode
-// every launch group is bracketed by an event pair labelled with its kernel.
-void Engine::record_frame(float frame_dt, hipStream_t s) {
+// VtClothSolverGPU::Simulate stage order (Velvet/VtClothSolverGPU.hpp:56-111) for one table set.
+// In timing mode every launch group is bracketed by an event pair labelled with its kernel.
+void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s) {
     const float h = frame_dt / (float)substeps_;
     stage_begin("k_frame_begin", s);
-    VTX_HIP(launch_frame_begin(ds_, frame_dt, h, s));
+    VTX_HIP(launch_frame_begin(d, frame_dt, h, s));
     stage_end(s);
     int cur = 0;
     for (int sub = 0; sub < substeps_; sub++) {
-        if (any_selfcol_ && sub % interleave_ == 0) hash_build(cur, s);
+        if (any_selfcol_ && sub % interleave_ == 0) hash_build(d, cur, s);
         if (need_collide_) {
             stage_begin("k_collide", s);
-            VTX_HIP(launch_collide(ds_, cur, h, tun_.gather_block, s));
+            VTX_HIP(launch_collide(d, cur, h, tun_.gather_block, s));
             stage_end(s);
             cur ^= 1;
         }
@@ -1312,21 +1373,37 @@ void Engine::record_frame(float frame_dt, hipStream_t s) {
         const int epi_last = last_sub ? EPILOGUE_FINALIZE : EPILOGUE_FINALIZE_PREDICT;
         if (iterations_ == 0) {
             stage_begin("k_iterate", s);
-            VTX_HIP(launch_iterate(ds_, cur, h, epi_last, false, tun_.lra_batch, s));
+            VTX_HIP(launch_iterate(d, cur, h, epi_last, false, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
         for (int it = 0; it < iterations_; it++) {
             const int epi = it == iterations_ - 1 ? epi_last : EPILOGUE_NONE;
             stage_begin("k_iterate", s);
-            VTX_HIP(launch_iterate(ds_, cur, h, epi, true, tun_.lra_batch, s));
+            VTX_HIP(launch_iterate(d, cur, h, epi, true, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
     }
     stage_begin("k_normals", s);
-    VTX_HIP(launch_normals(ds_, s));
+    VTX_HIP(launch_normals(d, s));
     stage_end(s);
+}
+
+// One frame: with two table sets the second runs on its own stream, forked from and joined
+// back into the solver's stream (also inside a graph capture), so the segments of the two
+// groups advance concurrently and every later operation on the solver's stream sees both.
+void Engine::record_frame(float frame_dt, hipStream_t s) {
+    if (groups_.size() < 2) {
+        record_group(groups_.empty() ? ds_ : groups_[0], frame_dt, s);
+        return;
+    }
+    VTX_HIP(hipEventRecord(ev_fork_, s));
+    VTX_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+    record_group(groups_[0], frame_dt, s);
+    record_group(groups_[1], frame_dt, stream2_);
+    VTX_HIP(hipEventRecord(ev_join_, stream2_));
+    VTX_HIP(hipStreamWaitEvent(s, ev_join_, 0));
 }
 
 void Engine::simulate(float frame_dt) {
@@ -1383,8 +1460,11 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
     VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
                 "gather_block must be 64, 128 or 256");
+    VTX_REQUIRE(t.streams == 1 || t.streams == 2, VTX_EINVAL, "streams must be 1 or 2");
+    if (t.streams != tun_.streams) tables_dirty_ = true;
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
+    for (DevState& g : groups_) g.remap = ds_.remap;
     invalidate_graph();
 }
 
@@ -1477,10 +1557,11 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         };
         bms = trial(best);
         // one knob at a time, every other value of it against the best so far
-        for (int knob = 0; knob < 5; knob++) {
+        for (int knob = 0; knob < 6; knob++) {
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
             if (knob == 4 && !need_collide_) continue;
+            if (knob == 5 && table_seg_.size() < 2) continue;
             const int nval = knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
@@ -1490,9 +1571,11 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
                 if (knob == 2) t.lra_batch = v == 0 ? 4 : v == 1 ? 16 : 64;
                 if (knob == 3) t.remap = v != 0;
                 if (knob == 4) t.gather_block = 64 << v;
+                if (knob == 5) t.streams = v + 1;
                 if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
                                 : knob == 2 ? t.lra_batch == base.lra_batch
-                                : knob == 3 ? t.remap == base.remap : t.gather_block == base.gather_block)
+                                : knob == 3 ? t.remap == base.remap
+                                : knob == 4 ? t.gather_block == base.gather_block : t.streams == base.streams)
                     continue;
                 const double ms = trial(t);
                 if (ms < 0.99 * bms) {
@@ -1529,6 +1612,7 @@ void Engine::set_stream(hipStream_t s) {
 }
 
 void Engine::set_timing(bool on) {
+    if (on != timing_) tables_dirty_ = true;   // timing mode runs all segments as one table set
     timing_ = on;
     timing_ms_.clear();
     timing_n_.clear();
@@ -1644,7 +1728,7 @@ void Engine::write_predicted(int32_t s, const float* xyz) {
 void Engine::run_hash() {
     ensure_committed();
     VTX_REQUIRE(any_selfcol_, VTX_ESTATE, "no active segment has self collision enabled");
-    hash_build(0, stream_);
+    for (const DevState& g : groups_) hash_build(g, 0, stream_);
     synchronize();
 }
 

@@ -120,6 +120,8 @@ struct Tuning {
     int lra_batch = 16;       // dense attachment distances loaded per batch: 4, 16 or 64
     bool remap = true;        // XCD-aware workgroup -> tile remap
     int gather_block = 256;   // threads per workgroup of the neighbour search and contact pass
+    int streams = 1;          // 2: segments split into two particle-balanced groups whose frames
+                              // run on two streams, overlapping latency- and VALU-bound stages
 };
 
 struct StageTime {
@@ -215,7 +217,8 @@ private:
     void rebuild_tables();
     void grow_pools(const Ranges& need);
     void record_frame(float frame_dt, hipStream_t s);
-    void hash_build(int in, hipStream_t s);
+    void hash_build(const DevState& d, int in, hipStream_t s);
+    void record_group(const DevState& d, float frame_dt, hipStream_t s);
     void stage_begin(const char* label, hipStream_t s);
     void stage_end(hipStream_t s);
     void check_seg(int32_t s) const;
@@ -249,7 +252,10 @@ private:
     uint32_t* part_ = nullptr;
     uint32_t* flags_ = nullptr;
 
-    DevState ds_{};
+    DevState ds_{};                          // every live segment
+    std::vector<DevState> groups_;           // the frame's table sets (1, or 2 on two streams)
+    hipStream_t stream2_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     std::vector<int32_t> table_seg_;         // table index -> segment id
     int passes_ = 1;
     int sorted_src_ = 0;

@@ -382,7 +382,7 @@ hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s) {
 
 hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_hist, dim3(d.nstiles), dim3(VTX_TILE), 0, s, d, src, shift);
-    hipError_t e = launch_scan(d.hist, d.nhist, d.part, s);
+    hipError_t e = launch_scan(d.hist + d.hbase0, d.nhist, d.part, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sort_scatter, dim3(d.nstiles), dim3(VTX_TILE), 0, s, d, src, shift);
     return hipGetLastError();

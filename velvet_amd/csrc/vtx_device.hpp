@@ -131,6 +131,7 @@ struct DevState {
     const uint32_t* gflags;   // grid: bending slots present at a cell (VTX_BX | VTX_BR | VTX_BD)
     const float4* brest4;     // grid: bending rest angles of a cell (X, R, D)
     uint32_t ntiles, nstiles, nhist, ncells;
+    uint32_t hbase0, pad1, pad2, pad3;       // first histogram entry of this table set
     uint32_t nparticles, nitiles, ngtiles, remap;   // remap: workgroups -> tiles through xcd_tile
 };
 
