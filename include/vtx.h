@@ -156,9 +156,10 @@ typedef struct vtx_tuning {
                                   64, 128 or 256 (each 256-particle tile split across workgroups) */
     int32_t streams;           /* 2: the segments split into two particle-balanced groups whose
                                   frames run concurrently on two streams; 1: one stream */
-    int32_t pad;
+    int32_t lra_split;         /* 1: grid cloths with 32 or more dense attachment slots run their
+                                  Jacobi pass with four threads per particle; 0: one thread */
 } vtx_tuning;
-int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);   /* vtx_set_tuning ignores frame_ms and pad */
+int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);   /* vtx_set_tuning ignores frame_ms */
 int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);
 /* Time the variants on the current live set (coordinate descent, `frames` timed frames each,
  * positions and velocities restored after every trial, so the simulation does not advance),

@@ -320,13 +320,22 @@ def test_config2_lra_frames_vs_fp64():
     assert err <= 1e-3 * sc.extent()
 
 
-@pytest.mark.parametrize("stencil", [True, False])
-def test_dense_and_listed_attachments_agree(stencil):
+def _lra40(frames=4):
+    """40 x 40 cloth whose first row (40 particles) is long-range attached: 40 dense slots, enough
+    for the four-threads-per-particle Jacobi pass on the stencil path."""
+    sc = S.config1(frames=frames)
+    c = S.grid_cloth(40, 0.05, center=(0.0, 1.5, 0.0), seed=0, stream=1, attached=tuple(range(40)))
+    return S.Scene(sc.params, [c], sc.colliders, frames, "lra40")
+
+
+@pytest.mark.parametrize("stencil,slots", [(True, 2), (False, 2), (True, 40), (False, 40)])
+def test_dense_and_listed_attachments_agree(stencil, slots):
     """Attachments registered slot-major over every particle are stored as distances only; the
     same attachments registered particle-major go through the per-particle lists. Each particle
-    sums its terms in slot order either way, so the trajectories are bit-identical."""
-    say(f"dense vs listed attachments stencil={stencil}")
-    sc = S.config1(frames=4)
+    sums its terms in four parts by slot index, each part in slot order, either way, so the
+    trajectories are bit-identical -- also when the dense form runs four threads per particle."""
+    say(f"dense vs listed attachments stencil={stencil} slots={slots}")
+    sc = S.config1(frames=4) if slots == 2 else _lra40()
     c = sc.cloths[0]
     order = np.lexsort((c.attach_slot, c.attach_pid))       # particle-major, slot order within
     c2 = S.Cloth(**{**c.__dict__, "attach_pid": c.attach_pid[order], "attach_slot": c.attach_slot[order],

@@ -73,9 +73,26 @@ def test_variants_bit_identical_long_range_attachments():
     say("variants on config 2 (256 dense slots)")
     sc = S.config2(frames=2)
     ref = _run([sc], 2)
-    for lb in (4, 64):
-        got = _run([sc], 2, lra_batch=lb, neighbor_order=0, xcd_remap=0)
+    for lb, split in ((4, 1), (64, 1), (4, 0), (16, 0), (64, 0)):
+        say(f"  lra_batch={lb} lra_split={split}")
+        got = _run([sc], 2, lra_batch=lb, lra_split=split, neighbor_order=0, xcd_remap=0)
         assert np.array_equal(got[0][0], ref[0][0]) and np.array_equal(got[0][1], ref[0][1])
+
+
+def test_lra_split_in_a_mixed_batch():
+    """A grid cloth with 40 dense slots (four threads per particle) batched with small cloths
+    (one thread per particle), on one or two streams: each cloth matches its own solo run."""
+    say("lra_split in a mixed batch")
+    big = S.grid_cloth(40, 0.05, center=(0.0, 1.5, 0.0), seed=0, stream=1, attached=tuple(range(40)))
+    small = _batch()[:3]
+    lra = S.Scene(small[0].params, [big], small[0].colliders, 3, "lra40")
+    scenes = [small[0], lra, small[1], small[2]]
+    alone = [_run([sc], 3)[0] for sc in scenes]
+    for v in (dict(), dict(lra_split=0), dict(streams=2), dict(streams=2, lra_split=0, lra_batch=4)):
+        say(f"  {v}")
+        got = _run(scenes, 3, **v)
+        for k in range(len(scenes)):
+            assert np.array_equal(got[k][0], alone[k][0]) and np.array_equal(got[k][1], alone[k][1]), (k, v)
 
 
 def test_autotune_keeps_state_and_caches():
@@ -122,3 +139,5 @@ def test_tuning_rejects_bad_values():
             s.set_tuning(gather_block=96)
         with pytest.raises(Exception):
             s.set_tuning(streams=3)
+        s.set_tuning(lra_split=0)
+        assert s.tuning()["lra_split"] == 0

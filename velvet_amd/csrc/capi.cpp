@@ -212,7 +212,7 @@ static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
     out->frame_ms = (float)ms;
     out->gather_block = t.gather_block;
     out->streams = t.streams;
-    out->pad = 0;
+    out->lra_split = t.lra_split ? 1 : 0;
 }
 
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
@@ -230,6 +230,7 @@ int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
     k.remap = t->xcd_remap != 0;
     k.gather_block = t->gather_block;
     k.streams = t->streams;
+    k.lra_split = t->lra_split != 0;
     s->engine.set_tuning(k);
     API_END
 }

@@ -1088,11 +1088,12 @@ void Engine::rebuild_tables() {
     free_tables();
     std::vector<SegDev> hseg;
     std::vector<TileDev> htile, hitile;
-    std::vector<GridTileDev> hgtile;
+    std::vector<GridTileDev> hgtile, hltile;
     std::vector<SortTileDev> hst;
     table_seg_.clear();
     any_selfcol_ = false;
     any_lra_dense_ = false;
+    any_lra_split_ = false;
     need_collide_ = false;
     live_particles_ = 0;
     uint32_t maxbits = 0, hbase = 0, compact = 0;
@@ -1109,7 +1110,7 @@ void Engine::rebuild_tables() {
     const bool split = tun_.streams == 2 && nlive >= 2 && !timing_;
     uint64_t cum = 0;
     bool in_second = false;
-    size_t t_split = 0, i_split = 0, g_split = 0, s_split = 0;
+    size_t t_split = 0, i_split = 0, g_split = 0, l_split = 0, s_split = 0;
     uint32_t h_split = 0, p_split = 0;
     for (size_t si = 0; si < segs_.size(); si++) {
         Segment* sgp = segs_[si].get();
@@ -1129,6 +1130,7 @@ void Engine::rebuild_tables() {
             t_split = htile.size();
             i_split = hitile.size();
             g_split = hgtile.size();
+            l_split = hltile.size();
             s_split = hst.size();
             h_split = hbase;
             p_split = (uint32_t)cum;
@@ -1146,6 +1148,8 @@ void Engine::rebuild_tables() {
         }
         if ((d.flags & VTX_SEG_SELFCOL) || d.ccount) need_collide_ = true;
         if (d.flags & VTX_SEG_LRA_DENSE) any_lra_dense_ = true;
+        const bool lsplit = sg.prep.grid && (d.flags & VTX_SEG_LRA_DENSE) && d.scount >= VTX_LRA_SPLIT_MIN;
+        any_lra_split_ = any_lra_split_ || lsplit;
         for (TileDev t : sg.prep.tiles) {
             t.seg = ti;
             t.begin += d.pbase;
@@ -1159,7 +1163,7 @@ void Engine::rebuild_tables() {
         for (GridTileDev gt : sg.prep.gtiles) {
             gt.seg = ti;
             gt.at_off += (uint32_t)sg.rg.at;
-            hgtile.push_back(gt);
+            (lsplit && tun_.lra_split ? hltile : hgtile).push_back(gt);
         }
         const uint32_t nst = (d.n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
         for (uint32_t k = 0; k < nst; k++) {
@@ -1200,6 +1204,7 @@ void Engine::rebuild_tables() {
     d.ntiles = (uint32_t)htile.size();
     d.nitiles = (uint32_t)hitile.size();
     d.ngtiles = (uint32_t)hgtile.size();
+    d.nltiles = (uint32_t)hltile.size();
     d.nstiles = (uint32_t)hst.size();
     d.nhist = hbase;
     d.ncells = (uint32_t)acell_.cap;
@@ -1207,6 +1212,7 @@ void Engine::rebuild_tables() {
     d.tiles = table_upload(table_allocs_, htile);
     d.itiles = table_upload(table_allocs_, hitile);
     d.gtiles = table_upload(table_allocs_, hgtile);
+    d.ltiles = table_upload(table_allocs_, hltile);
     d.stiles = table_upload(table_allocs_, hst);
     d.X = pl_.X;
     d.V = pl_.V;
@@ -1246,16 +1252,19 @@ void Engine::rebuild_tables() {
         g0.ntiles = (uint32_t)t_split;
         g0.nitiles = (uint32_t)i_split;
         g0.ngtiles = (uint32_t)g_split;
+        g0.nltiles = (uint32_t)l_split;
         g0.nstiles = (uint32_t)s_split;
         g0.nhist = h_split;
         g0.nparticles = p_split;
         g1.tiles = d.tiles + t_split;
         g1.itiles = d.itiles + i_split;
         g1.gtiles = d.gtiles + g_split;
+        g1.ltiles = d.ltiles + l_split;
         g1.stiles = d.stiles + s_split;
         g1.ntiles = d.ntiles - (uint32_t)t_split;
         g1.nitiles = d.nitiles - (uint32_t)i_split;
         g1.ngtiles = d.ngtiles - (uint32_t)g_split;
+        g1.nltiles = d.nltiles - (uint32_t)l_split;
         g1.nstiles = d.nstiles - (uint32_t)s_split;
         g1.hbase0 = h_split;
         g1.nhist = d.nhist - h_split;
@@ -1461,7 +1470,7 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
                 "gather_block must be 64, 128 or 256");
     VTX_REQUIRE(t.streams == 1 || t.streams == 2, VTX_EINVAL, "streams must be 1 or 2");
-    if (t.streams != tun_.streams) tables_dirty_ = true;
+    if (t.streams != tun_.streams || t.lra_split != tun_.lra_split) tables_dirty_ = true;
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
     for (DevState& g : groups_) g.remap = ds_.remap;
@@ -1481,7 +1490,7 @@ std::string Engine::shape_class() {
     std::ostringstream o;
     const uint32_t np = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, live_particles_), UINT32_MAX);
     o << "p" << ceil_log2(np) << ".s" << ceil_log2(std::max(1u, nseg))
-      << ".h" << (any_selfcol_ ? 1 : 0) << ".g" << (ngrid ? 1 : 0) << ".l" << (any_lra_dense_ ? 1 : 0) << "."
+      << ".h" << (any_selfcol_ ? 1 : 0) << ".g" << (ngrid ? 1 : 0) << ".l" << (any_lra_split_ ? 2 : any_lra_dense_ ? 1 : 0) << "."
       << substeps_ << "x" << iterations_;
     return o.str();
 }
@@ -1557,11 +1566,12 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         };
         bms = trial(best);
         // one knob at a time, every other value of it against the best so far
-        for (int knob = 0; knob < 6; knob++) {
+        for (int knob = 0; knob < 7; knob++) {
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
             if (knob == 4 && !need_collide_) continue;
             if (knob == 5 && table_seg_.size() < 2) continue;
+            if (knob == 6 && !any_lra_split_) continue;
             const int nval = knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
@@ -1572,10 +1582,12 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
                 if (knob == 3) t.remap = v != 0;
                 if (knob == 4) t.gather_block = 64 << v;
                 if (knob == 5) t.streams = v + 1;
+                if (knob == 6) t.lra_split = v != 0;
                 if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
                                 : knob == 2 ? t.lra_batch == base.lra_batch
                                 : knob == 3 ? t.remap == base.remap
-                                : knob == 4 ? t.gather_block == base.gather_block : t.streams == base.streams)
+                                : knob == 4 ? t.gather_block == base.gather_block
+                                : knob == 5 ? t.streams == base.streams : t.lra_split == base.lra_split)
                     continue;
                 const double ms = trial(t);
                 if (ms < 0.99 * bms) {

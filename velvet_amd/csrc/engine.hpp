@@ -122,6 +122,8 @@ struct Tuning {
     int gather_block = 256;   // threads per workgroup of the neighbour search and contact pass
     int streams = 1;          // 2: segments split into two particle-balanced groups whose frames
                               // run on two streams, overlapping latency- and VALU-bound stages
+    bool lra_split = true;    // grid segments with >= VTX_LRA_SPLIT_MIN dense slots: four threads
+                              // per particle in the Jacobi pass
 };
 
 struct StageTime {
@@ -266,6 +268,7 @@ private:
     uint64_t frames_ = 0;                    // frames simulated
     Tuning tun_;
     bool any_lra_dense_ = false;
+    bool any_lra_split_ = false;   // a stencil segment has >= VTX_LRA_SPLIT_MIN dense slots
     bool use_stencil_ = true;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;

@@ -264,13 +264,14 @@ __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
 }
 
 template <int ORDER, int BS>
-__global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __restrict__ tiles, int in) {
+__global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __restrict__ tiles,
+                                                  const GridTileDev* __restrict__ gtiles, int in) {
 #pragma clang fp contract(off)
     __shared__ uint32_t crange[VTX_NB_CACHE][BS];   // thread-private columns: no barrier needed
     uint32_t seg, i, tix, lane;
     tile_lane<VTX_TILE, BS>(d.remap, tix, lane);
     if (ORDER == NBR_PATCH) {
-        const GridTileDev t = d.gtiles[tix];
+        const GridTileDev t = gtiles[tix];
         seg = t.seg;
         const SegDev& s0 = d.segs[seg];
         const uint32_t a = t.a0 + lane / VTX_GT, b = t.b0 + lane % VTX_GT;
@@ -411,14 +412,16 @@ static void launch_neighbors_bs(const DevState& d, int in, int order, hipStream_
     constexpr uint32_t S = VTX_TILE / BS;
     const dim3 bl(BS);
     if (order == NBR_PARTICLE) {
-        hipLaunchKernelGGL((k_neighbors<NBR_PARTICLE, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, in);
+        hipLaunchKernelGGL((k_neighbors<NBR_PARTICLE, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, d.gtiles, in);
     } else if (order == NBR_SORTED) {
-        hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, in);
-    } else {   // grid segments by patches, the others in sorted order
+        hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.ntiles * S), bl, 0, s, d, d.tiles, d.gtiles, in);
+    } else {   // grid segments by patches (both grid tile lists), the others in sorted order
         if (d.ngtiles)
-            hipLaunchKernelGGL((k_neighbors<NBR_PATCH, BS>), dim3(d.ngtiles * S), bl, 0, s, d, d.itiles, in);
+            hipLaunchKernelGGL((k_neighbors<NBR_PATCH, BS>), dim3(d.ngtiles * S), bl, 0, s, d, d.itiles, d.gtiles, in);
+        if (d.nltiles)
+            hipLaunchKernelGGL((k_neighbors<NBR_PATCH, BS>), dim3(d.nltiles * S), bl, 0, s, d, d.itiles, d.ltiles, in);
         if (d.nitiles)
-            hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.nitiles * S), bl, 0, s, d, d.itiles, in);
+            hipLaunchKernelGGL((k_neighbors<NBR_SORTED, BS>), dim3(d.nitiles * S), bl, 0, s, d, d.itiles, d.gtiles, in);
     }
 }
 

@@ -234,60 +234,105 @@ __device__ __forceinline__ void stretch_term(V3 p, float w, float4 q, float rest
     }
 }
 
-// One long-range attachment (Velvet/VtClothSolverGPU.cu:218-234).
-// Branch-free (selects): the slot position is used on every path, so its loads can be issued
-// ahead of the term instead of inside a conditional block.
+// One long-range attachment (Velvet/VtClothSolverGPU.cu:218-234): with t the target distance,
+// a particle farther than t from its slot moves by -diff + diff / |diff| * t, which is
+// diff * (t / |diff| - 1); "|diff| > t" is the same test as "t / |diff| - 1 < 0", so one
+// reciprocal square root serves both. Branch-free (selects): the slot position is used on every
+// path, so its loads can be issued ahead of the term instead of inside a conditional block.
 __device__ __forceinline__ void attach_term(V3 p, float w, float4 slot, float dist0, float lrs, V3& delta,
                                             int& cnt) {
     const float target = dist0 * lrs;
     const V3 diff = p - xyz(slot);
-    const float dist = fsqrt(dot(diff, diff));
+    const float f = __builtin_fmaf(target, frsq(dot(diff, diff)), -1.0f);
     // the reference returns early for a pinned particle with a positive target
-    const bool act = !(w == 0.f && target > 0.f) && dist > target;
-    const V3 c = -diff + diff * (target * frcp(dist));
-    delta.x = act ? delta.x + c.x : delta.x;
-    delta.y = act ? delta.y + c.y : delta.y;
-    delta.z = act ? delta.z + c.z : delta.z;
+    const bool act = !(w == 0.f && target > 0.f) && f < 0.f;
+    const float k = act ? f : 0.f;
+    delta.x = __builtin_fmaf(diff.x, k, delta.x);
+    delta.y = __builtin_fmaf(diff.y, k, delta.y);
+    delta.z = __builtin_fmaf(diff.z, k, delta.z);
     cnt += act ? 1 : 0;
 }
 
-// All attachments of one particle, in registration order: dense (slot-major distances, the slot
-// position is the same for every lane) or from the tile's attach ELL.
+// Slot range [s0, s1) of part q of a segment's attachments (see VTX_LRA_PARTS).
+__device__ __forceinline__ void lra_part_range(const SegDev& sg, uint32_t q, uint32_t& s0, uint32_t& s1) {
+    const uint32_t qs = (sg.scount + VTX_LRA_PARTS - 1) / VTX_LRA_PARTS;
+    s0 = min(q * qs, sg.scount);
+    s1 = min(s0 + qs, sg.scount);
+}
+
+// Dense attachments of one particle to slots [s0, s1), summed from zero in slot order.
+// Distances are fetched LB slots at a time so that several loads are in flight per lane (a large
+// cloth may give only one wave per SIMD); every LB gives the same sum.
+template <int LB>
+__device__ __forceinline__ void attach_dense(const DevState& d, const SegDev& sg, uint32_t local, uint32_t s0,
+                                             uint32_t s1, V3 p, float w, V3& part, int& cnt) {
+    const float* __restrict__ lra = d.lra + sg.lbase + local;
+    const float4* __restrict__ slot = d.slots + sg.sbase;
+    const size_t n = sg.n;
+    uint32_t s = s0;
+    for (; s + LB <= s1; s += LB) {
+        float r[LB];
+#pragma unroll
+        for (int u = 0; u < LB; u++) r[u] = lra[(size_t)(s + u) * n];
+        // slot positions (wave-uniform) four at a time, so that few are live at once
+#pragma unroll
+        for (int g = 0; g < LB; g += 4) {
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) q[u] = slot[s + g + u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) attach_term(p, w, q[u], r[g + u], sg.lrs, part, cnt);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    for (; s < s1; s++) attach_term(p, w, slot[s], lra[(size_t)s * n], sg.lrs, part, cnt);
+}
+
+// All attachments of one particle, added to delta part by part (VTX_LRA_PARTS): dense
+// (slot-major distances, the slot position is the same for every lane) or from the tile's attach
+// ELL in registration order, where a part ends wherever the slot's part changes -- so attachments
+// registered particle-major in slot order sum exactly as the dense form does.
 template <int LB>
 __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg, uint32_t at_off, uint32_t at_k,
                                              uint32_t lane, uint32_t local, V3 p, float w, V3& delta, int& cnt) {
     if (sg.flags & VTX_SEG_LRA_DENSE) {
-        // Distances are fetched LB slots at a time so that several loads are in flight per lane
-        // (a large cloth may give only one wave per SIMD); the terms are still summed in slot
-        // order, so every LB gives the same result.
-        const float* __restrict__ lra = d.lra + sg.lbase + local;
-        const float4* __restrict__ slot = d.slots + sg.sbase;
-        const size_t n = sg.n;
-        uint32_t s = 0;
-        for (; s + LB <= sg.scount; s += LB) {
-            float r[LB];
-#pragma unroll
-            for (int u = 0; u < LB; u++) r[u] = lra[(size_t)(s + u) * n];
-            // slot positions (wave-uniform) four at a time, so that few are live at once
-#pragma unroll
-            for (int g = 0; g < LB; g += 4) {
-                float4 q[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) q[u] = slot[s + g + u];
-#pragma unroll
-                for (int u = 0; u < 4; u++) attach_term(p, w, q[u], r[g + u], sg.lrs, delta, cnt);
-                __builtin_amdgcn_sched_barrier(0);
+        if (sg.scount <= VTX_LRA_PARTS) {   // every slot is a part of its own
+            const float* __restrict__ lra = d.lra + sg.lbase + local;
+            const float4* __restrict__ slot = d.slots + sg.sbase;
+            for (uint32_t s = 0; s < sg.scount; s++) {
+                V3 part = v3(0.f, 0.f, 0.f);
+                attach_term(p, w, slot[s], lra[(size_t)s * sg.n], sg.lrs, part, cnt);
+                delta += part;
             }
+            return;
         }
-        for (; s < sg.scount; s++) attach_term(p, w, slot[s], lra[(size_t)s * n], sg.lrs, delta, cnt);
+#pragma unroll 1
+        for (uint32_t q = 0; q < VTX_LRA_PARTS; q++) {
+            uint32_t s0, s1;
+            lra_part_range(sg, q, s0, s1);
+            V3 part = v3(0.f, 0.f, 0.f);
+            attach_dense<LB>(d, sg, local, s0, s1, p, w, part, cnt);
+            delta += part;
+        }
         return;
     }
+    const uint32_t qs = (sg.scount + VTX_LRA_PARTS - 1) / VTX_LRA_PARTS;
     const uint2* __restrict__ at = d.at + at_off + lane;
+    static_assert(VTX_LRA_PARTS == 4, "the part of a listed slot is found with three compares");
+    V3 part = v3(0.f, 0.f, 0.f);
+    uint32_t cur = 0;
     for (uint32_t k = 0; k < at_k; k++) {
         const uint2 e = at[(size_t)k * VTX_TILE];
         if (e.x == VTX_INVALID) continue;
-        attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, delta, cnt);
+        const uint32_t q = (e.x >= qs ? 1u : 0u) + (e.x >= 2 * qs ? 1u : 0u) + (e.x >= 3 * qs ? 1u : 0u);
+        if (q != cur) {
+            delta += part;
+            part = v3(0.f, 0.f, 0.f);
+            cur = q;
+        }
+        attach_term(p, w, d.slots[sg.sbase + e.x], __uint_as_float(e.y), sg.lrs, part, cnt);
     }
+    delta += part;
 }
 
 // The dihedral bending constraint (Velvet/VtClothSolverGPU.cu:128-177), evaluated through the
@@ -476,16 +521,20 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
 
 // One slot type: evaluate its slots (t, t + 256), then every tile vertex adds its four
 // memberships of this type in role order.
-template <int TYPE, bool COLMAJOR>
+template <int TYPE, bool COLMAJOR, int BT>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
                                                float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
                                                V3& delta, int& cnt) {
     constexpr int NS = TYPE == 0 ? VTX_NSX : TYPE == 1 ? VTX_NSR : VTX_NSD;
     const uint32_t tid = threadIdx.x;
-    eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
-    if (tid < NS - VTX_TILE)
-        eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, VTX_TILE + tid, xpbd);
+    if (BT >= NS) {   // one slot per thread
+        if (tid < NS) eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
+    } else {          // two rounds
+        static_assert(2 * BT >= NS, "a slot type fits two rounds of the workgroup");
+        eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
+        if (tid < NS - BT) eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, BT + tid, xpbd);
+    }
     __syncthreads();
     if (inside) {
         // role r of the slot owned at (ty + dy_r, tx + dx_r), per type:
@@ -518,20 +567,29 @@ __device__ __forceinline__ float4 nan4() {
     return make_float4(q, q, q, q);
 }
 
-template <int EPI, int LB>
-__global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, float h) {
+// SPLIT = 1: one thread per vertex. SPLIT = VTX_LRA_PARTS: a workgroup of four threads per
+// vertex for segments with many dense attachments (a 256 x 256 cloth with 256 slots gives one
+// tile per CU, and a single wave per SIMD would walk 256 slots per vertex): thread q of a vertex
+// sums attachment part q, the first thread adds the parts in order, and all threads share the
+// bending slots (one round per type). Both forms add the same terms in the same order.
+template <int EPI, int LB, int SPLIT>
+__global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, const GridTileDev* __restrict__ tiles,
+                                                                   int in, float h) {
+    constexpr int BT = VTX_TILE * SPLIT;
     __shared__ float4 sp[VTX_GP * VTX_GP];
     __shared__ float corr[12][VTX_NSMAX];          // plane r*3+c: component c of role r
     __shared__ uint8_t act[VTX_NSMAX];
-    const GridTileDev t = d.gtiles[tile_of(d.remap)];
+    const GridTileDev t = tiles[tile_of(d.remap)];
     const SegDev sg = d.segs[t.seg];
     const int rows = (int)sg.rows, cols = (int)sg.cols;
     const int a0 = (int)t.a0, b0 = (int)t.b0;
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = SPLIT == 1 ? tid : tid % VTX_TILE;
+    const uint32_t qpart = SPLIT == 1 ? 0u : tid / VTX_TILE;
     const float4* __restrict__ P = d.P[in];
 
     // stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
-    for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += VTX_TILE) {
+    for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += BT) {
         const int a = a0 - VTX_GH + (int)(e / VTX_GP), b = b0 - VTX_GH + (int)(e % VTX_GP);
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (a >= 0 && a < rows && b >= 0 && b < cols) v = P[sg.pbase + (uint32_t)(a * cols + b)];
@@ -539,10 +597,11 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
     }
     __syncthreads();
 
-    const int ty = (int)(tid / VTX_GT), tx = (int)(tid % VTX_GT);
+    const int ty = (int)(lane / VTX_GT), tx = (int)(lane % VTX_GT);
     const int a = a0 + ty, b = b0 + tx;
-    const bool inside = a < rows && b < cols;
-    const uint32_t g = inside ? sg.pbase + (uint32_t)(a * cols + b) : 0u;
+    const bool in_cloth = a < rows && b < cols;
+    const bool inside = in_cloth && qpart == 0;        // the thread that owns the vertex
+    const uint32_t g = in_cloth ? sg.pbase + (uint32_t)(a * cols + b) : 0u;
     const int sc = (ty + VTX_GH) * VTX_GP + (tx + VTX_GH);    // staged index of this vertex
     const float4 p4 = sp[sc];
     V3 p = xyz(p4);
@@ -565,18 +624,40 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate_grid(DevState d, int in, f
         if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
         if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
         if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
-        attach_terms<LB>(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
+        if constexpr (SPLIT == 1) attach_terms<LB>(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
+    }
+    if constexpr (SPLIT > 1) {   // only dense segments come here (VTX_SEG_LRA_DENSE)
+        __shared__ float4 lpart[SPLIT - 1][VTX_TILE];
+        V3 part = v3(0.f, 0.f, 0.f);
+        int pc = 0;
+        if (in_cloth) {
+            uint32_t s0, s1;
+            lra_part_range(sg, qpart, s0, s1);
+            attach_dense<LB>(d, sg, (uint32_t)(a * cols + b), s0, s1, p, w, part, pc);
+            if (qpart > 0) lpart[qpart - 1][lane] = f4(part, __int_as_float(pc));
+        }
+        __syncthreads();
+        if (inside) {
+            delta += part;
+            cnt += pc;
+#pragma unroll
+            for (int q = 0; q < SPLIT - 1; q++) {
+                const float4 e = lpart[q][lane];
+                delta += xyz(e);
+                cnt += __float_as_int(e.w);
+            }
+        }
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
     if (t.order == 0) {   // tile-uniform: one branch, each order compiled on its own
-        bend_type_pass<0, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<1, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<2, false>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<0, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<1, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<2, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
     } else {
-        bend_type_pass<0, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<1, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<2, true>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<0, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<1, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+        bend_type_pass<2, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
     }
     if (!inside) return;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
@@ -639,15 +720,22 @@ hipError_t launch_collide(const DevState& d, int in, float h, int block, hipStre
     return last_launch();
 }
 
+template <int LB, int SPLIT>
+static void launch_iterate_grid(const DevState& d, const GridTileDev* tiles, uint32_t ntiles, int in, float h,
+                                int epilogue, hipStream_t s) {
+    if (!ntiles) return;
+    const dim3 gg(ntiles), bl(VTX_TILE * SPLIT);
+    if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate_grid<EPI_NONE, LB, SPLIT>), gg, bl, 0, s, d, tiles, in, h);
+    else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE, LB, SPLIT>), gg, bl, 0, s, d, tiles, in, h);
+    else hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE_PREDICT, LB, SPLIT>), gg, bl, 0, s, d, tiles, in, h);
+}
+
 template <int LB>
 static void launch_iterate_lb(const DevState& d, int in, float h, int epilogue, hipStream_t s) {
     dim3 bl(VTX_TILE);
-    if (d.ngtiles) {
-        dim3 gg(d.ngtiles);
-        if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate_grid<EPI_NONE, LB>), gg, bl, 0, s, d, in, h);
-        else if (epilogue == EPI_FINALIZE) hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE, LB>), gg, bl, 0, s, d, in, h);
-        else hipLaunchKernelGGL((k_iterate_grid<EPI_FINALIZE_PREDICT, LB>), gg, bl, 0, s, d, in, h);
-    }
+    launch_iterate_grid<LB, 1>(d, d.gtiles, d.ngtiles, in, h, epilogue, s);
+    // four threads per particle leave 128 registers per thread: batches of 64 would spill
+    launch_iterate_grid<(LB > 16 ? 16 : LB), VTX_LRA_PARTS>(d, d.ltiles, d.nltiles, in, h, epilogue, s);
     if (d.nitiles) {
         dim3 gr(d.nitiles);
         if (epilogue == EPI_NONE) hipLaunchKernelGGL((k_iterate<EPI_NONE, true, LB>), gr, bl, 0, s, d, d.itiles, in, h);

@@ -35,6 +35,13 @@
 // Attachments are "every particle to every slot, in slot order" (Velvet/VtClothObjectGPU.hpp:134-148):
 // only the distances are stored, slot-major: lra[lbase + slot * n + particle].
 #define VTX_SEG_LRA_DENSE 512u
+// A particle's attachment terms are summed in four parts by slot index -- slots
+// [q*Qs, (q+1)*Qs) with Qs = ceil(scount / 4) -- each part from zero in slot order, and the parts
+// are added to the particle's delta in order q = 0..3. One lane can run all four parts, or four
+// lanes one part each (stencil segments with at least VTX_LRA_SPLIT_MIN dense slots), with the
+// same result.
+#define VTX_LRA_PARTS 4
+#define VTX_LRA_SPLIT_MIN 32u
 
 // 2-D grid tiles of the stencil path: VTX_GT x VTX_GT particles, halo of 2 rows/cols.
 #define VTX_GT 16
@@ -127,11 +134,12 @@ struct DevState {
     uint32_t* nbr;            // neighbour tables, column-major per segment
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
     const GridTileDev* gtiles;
+    const GridTileDev* ltiles;   // grid tiles whose dense attachments run on four lanes per particle
     const float4* rest4;      // grid: stretch rest lengths owned by a vertex (R, D, X-diag, anti-diag)
     const uint32_t* gflags;   // grid: bending slots present at a cell (VTX_BX | VTX_BR | VTX_BD)
     const float4* brest4;     // grid: bending rest angles of a cell (X, R, D)
     uint32_t ntiles, nstiles, nhist, ncells;
-    uint32_t hbase0, pad1, pad2, pad3;       // first histogram entry of this table set
+    uint32_t hbase0, nltiles, pad2, pad3;    // first histogram entry of this table set
     uint32_t nparticles, nitiles, ngtiles, remap;   // remap: workgroups -> tiles through xcd_tile
 };
 

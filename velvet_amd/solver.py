@@ -191,7 +191,7 @@ class ClothSolver:
         check(lib().vtx_set_use_graph(self.h, int(bool(on))))
 
     # Launch variants (all give bit-identical results) and the tuner that picks among them.
-    _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap", "gather_block", "streams")
+    _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap", "gather_block", "streams", "lra_split")
 
     def tuning(self) -> dict:
         t = VtxTuning()
@@ -204,7 +204,7 @@ class ClothSolver:
         if unknown:
             raise KeyError(f"unknown tuning keys {sorted(unknown)}")
         cur.update({k: int(v) for k, v in kw.items()})
-        t = VtxTuning(**cur, frame_ms=0.0, pad=0)
+        t = VtxTuning(**cur, frame_ms=0.0)
         check(lib().vtx_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, frames: int = 3, frame_dt: Optional[float] = None, use_cache: bool = False) -> dict:
