@@ -167,23 +167,46 @@ def cpu_baseline(scenes, budget_s):
     return done_ps / spent, spent, frames, cloths
 
 
-def serve_run(rate, seconds, seed, max_live_particles, use_graph, step_budget_ms=0.0, ahead=4):
+def parse_tuning(text):
+    """'k=v,k=v' -> {k: int(v)}"""
+    return {k: int(v) for k, v in (kv.split("=") for kv in text.split(",") if kv)}
+
+
+def serve_pool(seed, count=64):
+    """Job descriptions (scene, horizon) the serving run's requests cycle through: sizes
+    log-uniform 32^2..512^2, horizons U(30, 600), attachments and colliders as drawn per job.
+    Building a cloth is the client's work, so it happens before the timed run."""
+    return [S.serving_job(seed, k) for k in range(count)]
+
+
+def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms=0.0, ahead=4,
+              reserve_particles=6_000_000, tuning=None):
     """Poisson arrivals at `rate` req/s for `seconds` of wall time against the serving loop.
 
-    Arrived requests wait in a backlog (descriptors only); the driver materialises and submits
-    them while the server's own queue holds fewer than `ahead` jobs, so host memory stays
-    bounded when arrivals exceed capacity. Step latency is the server's own wall time per step.
+    Request k is job description pool[k % len(pool)]. Arrived requests wait in a backlog; the
+    loop submits them (the server packs each on the host) while the server's own queue holds
+    fewer than `ahead` jobs, so host memory stays bounded when arrivals exceed capacity. Step
+    latency is the server's own wall time per step;
+    admission wait is the time from a request's arrival to the end of the step that admitted it
+    (requests still waiting at the end are counted in `backlog`, not in the wait percentiles).
+    Finished jobs are read back by the server and released here.
     """
     from velvet_amd.server import Server
-    rng = np.random.default_rng(seed)
-    lat, work, live = [], 0, []
+    rng = np.random.default_rng(0)
+    lat, work, live, waits = [], 0, [], []
+    phases = {"admit": [], "frame": [], "retire": []}
     backlog = []
+    arrival = {}
     arrived = submitted = admitted = completed = 0
     k = 0
     next_t = rng.exponential(1.0 / rate)
     with Server(max_live_particles=max_live_particles, use_graph=use_graph,
-                step_budget_ms=step_budget_ms) as sv:
+                step_budget_ms=step_budget_ms, reserve_particles=reserve_particles) as sv:
+        if tuning:
+            sv.solver.set_tuning(**tuning)
+        tun = sv.solver.tuning()
         queued = 0
+        waiting, running = {}, set()
         t0 = time.perf_counter()
         while True:
             now = time.perf_counter() - t0
@@ -191,30 +214,48 @@ def serve_run(rate, seconds, seed, max_live_particles, use_graph, step_budget_ms
                 break
             while next_t <= now:
                 backlog.append(k)
+                arrival[k] = next_t
                 k += 1
                 arrived += 1
                 next_t += rng.exponential(1.0 / rate)
             while backlog and queued < ahead:
-                sc, horizon = S.serving_job(seed, backlog.pop(0))
-                sv.submit(sc, horizon)
+                r = backlog.pop(0)
+                sc, horizon = pool[r % len(pool)]
+                waiting[sv.submit(sc, horizon)] = arrival.pop(r)
                 submitted += 1
                 queued += 1
             st = sv.step()
+            t_end = time.perf_counter() - t0
             queued = st.queued_jobs
             admitted += st.admitted
             completed += st.completed
+            if st.admitted:
+                for j in [j for j in waiting if sv.status(j)[0] in (2, 3, 4)]:   # running, done, failed
+                    waits.append((t_end - waiting.pop(j)) * 1e3)
+                    running.add(j)
+            if st.completed:
+                for j in [j for j in running if sv.status(j)[0] in (3, 4)]:
+                    sv.release(j)
+                    running.discard(j)
             if st.live_jobs or st.completed:
                 lat.append(st.step_ms)
+                phases["admit"].append(st.admit_ms)
+                phases["frame"].append(st.frame_ms)
+                phases["retire"].append(st.retire_ms)
                 work += st.particle_substeps
                 live.append(st.live_particles)
         wall = time.perf_counter() - t0
+    pct = lambda a, q: float(np.percentile(a, q)) if a else None
     return {"rate_rps": rate, "seconds": round(wall, 2), "steps": len(lat),
-            "p50_ms": float(np.percentile(lat, 50)) if lat else None,
-            "p99_ms": float(np.percentile(lat, 99)) if lat else None,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "particle_substeps_per_s": work / wall, "arrived": arrived, "admitted": admitted,
-            "completed": completed, "backlog": len(backlog) + queued,
+            "completed": completed, "backlog": len(backlog) + queued, "distinct_jobs": len(pool),
+            "admit_wait_p50_ms": pct(waits, 50), "admit_wait_p99_ms": pct(waits, 99),
+            "phase_p50_ms": {k: pct(v, 50) for k, v in phases.items()},
+            "phase_p99_ms": {k: pct(v, 99) for k, v in phases.items()},
             "mean_live_particles": float(np.mean(live)) if live else 0.0,
-            "max_live_particles_cap": max_live_particles, "step_budget_ms": step_budget_ms}
+            "max_live_particles_cap": max_live_particles, "step_budget_ms": step_budget_ms,
+            "reserve_particles": reserve_particles, "tuning": tun}
 
 
 def main():
@@ -233,12 +274,14 @@ def main():
                     help="fix launch variants instead of tuning, e.g. use_graph=1,neighbor_order=1")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-rates", default="50", help="comma-separated Poisson rates (req/s)")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-rates", default="50,100,200,400", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-cap", type=int, default=0, help="live-particle cap; 0 = none")
     ap.add_argument("--serve-budget-ms", type=float, default=11.0,
                     help="admission budget on the predicted frame time")
     ap.add_argument("--serve-graph", type=int, default=0)
+    ap.add_argument("--serve-tuning", default="", metavar="K=V,...",
+                    help="launch variants of the serving solver, e.g. streams=2")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
@@ -259,7 +302,7 @@ def main():
         solver.load_scene(sc)
     tuning = solver.tuning()
     if args.set_tuning:
-        solver.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.set_tuning.split(","))})
+        solver.set_tuning(**parse_tuning(args.set_tuning))
         tuning = solver.tuning()
     elif args.tune_frames > 0:
         log(f"tuning launch variants ({args.tune_frames} frames each)")
@@ -303,10 +346,12 @@ def main():
     serving = None
     if not args.no_serve:
         serving = []
+        log("serving: building the job descriptions")
+        pool = serve_pool(0)
         for r in [float(x) for x in args.serve_rates.split(",") if x]:
             log(f"serving: {r:g} req/s for {args.serve_seconds:g} s, cap {args.serve_cap} live particles")
-            serving.append(serve_run(r, args.serve_seconds, 0, args.serve_cap, bool(args.serve_graph),
-                                     args.serve_budget_ms))
+            serving.append(serve_run(r, args.serve_seconds, pool, args.serve_cap, bool(args.serve_graph),
+                                     args.serve_budget_ms, tuning=parse_tuning(args.serve_tuning)))
             log(f"  {serving[-1]}")
 
     cpu = None

@@ -268,6 +268,9 @@ typedef struct vtx_server_config {
     int32_t check_every;           /* non-finite check period in steps; 0 = never */
     float step_budget_ms;          /* admit while the predicted frame time stays within it (cost
                                       per particle-substep measured on the live steps); 0 = none */
+    uint64_t reserve_particles;    /* size the device pools at creation for this many live
+                                      particles of grid cloths, so admission never waits for a
+                                      pool to grow; 0 = grow on demand */
 } vtx_server_config;
 
 typedef struct vtx_server_stats {
@@ -275,6 +278,9 @@ typedef struct vtx_server_stats {
     uint64_t live_particles;
     uint64_t particle_substeps;    /* work done by this step */
     double step_ms;                /* wall time of this step (admission + frame + retirement) */
+    double admit_ms;               /* of which: uploading admitted jobs and rebuilding the tables */
+    double frame_ms;               /* of which: the frame of every live job */
+    double retire_ms;              /* of which: reading back and removing finished jobs */
 } vtx_server_stats;
 
 typedef struct vtx_server vtx_server;

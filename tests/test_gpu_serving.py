@@ -42,11 +42,13 @@ def alone(scene, frames):
         return s.positions()
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_mix_equals_alone(use_graph):
-    say(f"serving mix vs alone graph={use_graph}")
+@pytest.mark.parametrize("use_graph,reserve", [(False, 0), (True, 0), (False, 200_000)])
+def test_mix_equals_alone(use_graph, reserve):
+    """Jobs that arrive while others run are uploaded during a frame (staged) and join at the
+    next step; every result equals the job's solo run, with or without pre-sized pools."""
+    say(f"serving mix vs alone graph={use_graph} reserve={reserve}")
     js = jobs(5)
-    with Server(use_graph=use_graph) as sv:
+    with Server(use_graph=use_graph, reserve_particles=reserve) as sv:
         ids = []
         # staggered arrivals: two now, one after 2 steps, two after 5
         arrivals = {0: [0, 1], 2: [2], 5: [3, 4]}
@@ -56,6 +58,8 @@ def test_mix_equals_alone(use_graph):
                 ids.append((k, sv.submit(js[k][0], js[k][1])))
             st = sv.step()
             step += 1
+            assert min(st.admit_ms, st.frame_ms, st.retire_ms) >= 0.0
+            assert st.frame_ms <= st.step_ms and st.retire_ms <= st.step_ms
             if len(ids) == 5 and all(sv.status(j)[0] == DONE for _, j in ids):
                 break
         for k, j in ids:

@@ -35,7 +35,7 @@ class VtxCollider(C.Structure):
 class VtxServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("hbm_reserve_bytes", C.c_uint64),
                 ("max_live_particles", C.c_uint64), ("max_live_jobs", u32), ("use_graph", i32),
-                ("check_every", i32), ("step_budget_ms", f32)]
+                ("check_every", i32), ("step_budget_ms", f32), ("reserve_particles", C.c_uint64)]
 
 
 class VtxTuning(C.Structure):
@@ -46,7 +46,8 @@ class VtxTuning(C.Structure):
 class VtxServerStats(C.Structure):
     _fields_ = [("live_jobs", u32), ("queued_jobs", u32), ("admitted", u32), ("completed", u32),
                 ("failed", u32), ("pad", u32), ("live_particles", C.c_uint64),
-                ("particle_substeps", C.c_uint64), ("step_ms", C.c_double)]
+                ("particle_substeps", C.c_uint64), ("step_ms", C.c_double), ("admit_ms", C.c_double),
+                ("frame_ms", C.c_double), ("retire_ms", C.c_double)]
 
 
 _SIGS = {

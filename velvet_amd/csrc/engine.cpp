@@ -276,6 +276,10 @@ Engine::~Engine() {
         (void)hipStreamSynchronize(stream2_);
         (void)hipStreamDestroy(stream2_);
     }
+    if (cstream_) {
+        (void)hipStreamSynchronize(cstream_);
+        (void)hipStreamDestroy(cstream_);
+    }
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -926,6 +930,42 @@ void Engine::grow_pools(const Ranges& need) {
     grow(alra_, need.n_lra, [&](size_t o, size_t c) { regrow(pl_.lra, o, c); });
 }
 
+// Whether every pool can take `need` without growing.
+bool Engine::pools_fit(const Ranges& need) {
+    auto fits = [](RangeAlloc& a, size_t n) {
+        size_t off;
+        if (!n) return true;
+        if (!a.take(n, &off)) return false;
+        a.give(off, n);
+        return true;
+    };
+    return fits(ap_, need.n_p) && fits(acell_, need.n_cell) && fits(anbr_, need.n_nbr) && fits(ast_, need.n_st) &&
+           fits(aat_, need.n_at) && fits(abd_, need.n_bd) && fits(atr_, need.n_tr) && fits(abrec_, need.n_brec) &&
+           fits(atrec_, need.n_trec) && fits(aslot_, need.n_slot) && fits(acol_, need.n_col) && fits(alra_, need.n_lra);
+}
+
+void Engine::stage(int32_t s) {
+    Segment& sg = seg(s);
+    VTX_REQUIRE(!sg.active, VTX_ESTATE, "only an inactive segment can be staged");
+    if (sg.uploaded) return;
+    prepare(s);
+    if (!pools_fit(needs_of(sg))) VTX_HIP(hipStreamSynchronize(stream_));   // growth moves every pool
+    if (!cstream_) VTX_HIP(hipStreamCreateWithFlags(&cstream_, hipStreamNonBlocking));
+    upload(sg, cstream_);
+}
+
+void Engine::reserve(uint64_t particles) {
+    if (!particles) return;
+    VTX_HIP(hipStreamSynchronize(stream_));
+    Ranges need;
+    need.n_p = particles;
+    need.n_cell = 2 * particles;                              // default table size 2 n
+    need.n_nbr = 64 * particles;                              // default neighbour cap
+    need.n_tr = 8 * particles;                                // triangle lists, 6 per vertex + padding
+    need.n_trec = 2 * particles;                              // 2 triangles per vertex
+    grow_pools(need);
+}
+
 void Engine::free_pools() {
     void* ptrs[] = {pl_.X, pl_.V, pl_.P0, pl_.P1, pl_.X0, pl_.N, pl_.sP, pl_.sX0, pl_.rest4, pl_.brest4,
                     pl_.k0, pl_.k1, pl_.v0, pl_.v1, pl_.gflags, pl_.cells, pl_.nbr, pl_.st, pl_.at, pl_.bd,
@@ -941,7 +981,7 @@ static void h2d(T* dst, const std::vector<T>& src, hipStream_t s) {
 }
 
 // Copy one prepared segment into freshly taken pool ranges.
-void Engine::upload(Segment& sg) {
+void Engine::upload(Segment& sg, hipStream_t s) {
     const Ranges need = needs_of(sg);
     grow_pools(need);
     Ranges r = need;
@@ -1010,17 +1050,32 @@ void Engine::upload(Segment& sg) {
     d.cols = pr.gcols;
     sg.dev = d;
 
-    hipStream_t s = stream_;
+    // Host-to-device bytes are the cost of admitting a segment: copies that repeat data already
+    // on the device are made there (predicted positions, and initial positions when they equal
+    // the positions), zero velocities are cleared, and all-zero rest angles are not read.
+    const size_t pbytes = sizeof(float4) * n;
     h2d(pl_.X + r.p, pr.x, s);
-    h2d(pl_.V + r.p, pr.v, s);
-    h2d(pl_.P0 + r.p, pr.x, s);
-    h2d(pl_.P1 + r.p, pr.x, s);
-    h2d(pl_.X0 + r.p, pr.x0, s);
-    if (n) VTX_HIP(hipMemsetAsync(pl_.N + r.p, 0, sizeof(float4) * n, s));
+    if (n) {
+        VTX_HIP(hipMemcpyAsync(pl_.P0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+        VTX_HIP(hipMemcpyAsync(pl_.P1 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+        // kernels read only xyz of the initial positions, so equal xyz lets X stand in
+        bool same = pr.x0.size() == pr.x.size();
+        for (uint32_t i = 0; same && i < n; i++) same = std::memcmp(&pr.x0[i], &pr.x[i], 12) == 0;
+        if (same)
+            VTX_HIP(hipMemcpyAsync(pl_.X0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+        else
+            h2d(pl_.X0 + r.p, pr.x0, s);
+        const bool v0 = std::all_of(pr.v.begin(), pr.v.end(), [](const float4& v) {
+            return v.x == 0.f && v.y == 0.f && v.z == 0.f && v.w == 0.f;
+        });
+        if (v0) VTX_HIP(hipMemsetAsync(pl_.V + r.p, 0, pbytes, s));
+        else h2d(pl_.V + r.p, pr.v, s);
+        VTX_HIP(hipMemsetAsync(pl_.N + r.p, 0, pbytes, s));
+    }
     if (pr.grid) {
         h2d(pl_.rest4 + r.p, pr.rest4, s);
         h2d(pl_.gflags + r.p, pr.gflags, s);
-        h2d(pl_.brest4 + r.p, pr.brest4, s);
+        if (!(pr.grid_flags & VTX_SEG_BEND_REST0)) h2d(pl_.brest4 + r.p, pr.brest4, s);
     }
     if (r.n_cell) VTX_HIP(hipMemsetAsync(pl_.cells + r.cell, 0xFF, sizeof(uint2) * r.n_cell, s));
     if (r.n_nbr) VTX_HIP(hipMemsetAsync(pl_.nbr + r.nbr, 0xFF, sizeof(uint32_t) * r.n_nbr, s));
@@ -1057,22 +1112,49 @@ void Engine::release(Segment& sg) {
 }
 
 void Engine::free_tables() {
-    for (void* p : table_allocs_) (void)hipFree(p);
-    table_allocs_.clear();
+    if (tab_dev_) (void)hipFree(tab_dev_);
+    if (tab_host_) (void)hipHostFree(tab_host_);
+    tab_dev_ = tab_host_ = nullptr;
+    tab_cap_ = 0;
 }
 
-template <class T>
-static T* table_upload(std::vector<void*>& allocs, const std::vector<T>& h) {
-    void* p = nullptr;
-    const size_t bytes = std::max<size_t>(sizeof(T) * h.size(), 256);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(VTX_ENOMEM, "hipMalloc of a table failed");
+// The launch tables live in one device arena, filled from one pinned staging buffer by one
+// asynchronous copy on the solver's stream; both are reused across rebuilds and only grow.
+// Callers have synchronized the stream, so neither buffer is still in use.
+struct TableSet {
+    std::vector<std::pair<const void*, size_t>> src;
+    std::vector<size_t> off;
+    size_t total = 0;
+    template <class T>
+    size_t add(const std::vector<T>& h) {
+        const size_t o = (total + 255) & ~(size_t)255;
+        src.emplace_back(h.data(), sizeof(T) * h.size());
+        off.push_back(o);
+        total = o + sizeof(T) * h.size();
+        return src.size() - 1;
     }
-    allocs.push_back(p);
-    if (!h.empty()) VTX_HIP(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-    return (T*)p;
+};
+
+void Engine::upload_tables(const TableSet& t) {
+    const size_t bytes = std::max<size_t>(t.total, 256);
+    if (bytes > tab_cap_) {
+        free_tables();
+        const size_t cap = bytes + bytes / 2;
+        if (hipMalloc((void**)&tab_dev_, cap) != hipSuccess) {
+            (void)hipGetLastError();
+            tab_dev_ = nullptr;
+            throw Error(VTX_ENOMEM, "hipMalloc of the launch tables failed");
+        }
+        if (hipHostMalloc((void**)&tab_host_, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            tab_host_ = nullptr;
+            throw Error(VTX_ENOMEM, "pinned staging of the launch tables failed");
+        }
+        tab_cap_ = cap;
+    }
+    for (size_t k = 0; k < t.src.size(); k++)
+        if (t.src[k].second) std::memcpy(tab_host_ + t.off[k], t.src[k].first, t.src[k].second);
+    VTX_HIP(hipMemcpyAsync(tab_dev_, tab_host_, t.total, hipMemcpyHostToDevice, stream_));
 }
 
 static uint32_t ceil_log2(uint32_t v) {
@@ -1085,7 +1167,6 @@ static uint32_t ceil_log2(uint32_t v) {
 void Engine::rebuild_tables() {
     VTX_HIP(hipStreamSynchronize(stream_));
     invalidate_graph();
-    free_tables();
     std::vector<SegDev> hseg;
     std::vector<TileDev> htile, hitile;
     std::vector<GridTileDev> hgtile, hltile;
@@ -1208,12 +1289,16 @@ void Engine::rebuild_tables() {
     d.nstiles = (uint32_t)hst.size();
     d.nhist = hbase;
     d.ncells = (uint32_t)acell_.cap;
-    d.segs = table_upload(table_allocs_, hseg);
-    d.tiles = table_upload(table_allocs_, htile);
-    d.itiles = table_upload(table_allocs_, hitile);
-    d.gtiles = table_upload(table_allocs_, hgtile);
-    d.ltiles = table_upload(table_allocs_, hltile);
-    d.stiles = table_upload(table_allocs_, hst);
+    TableSet ts;
+    const size_t k_seg = ts.add(hseg), k_tile = ts.add(htile), k_itile = ts.add(hitile), k_gtile = ts.add(hgtile),
+                 k_ltile = ts.add(hltile), k_st = ts.add(hst);
+    upload_tables(ts);
+    d.segs = (const SegDev*)(tab_dev_ + ts.off[k_seg]);
+    d.tiles = (const TileDev*)(tab_dev_ + ts.off[k_tile]);
+    d.itiles = (const TileDev*)(tab_dev_ + ts.off[k_itile]);
+    d.gtiles = (const GridTileDev*)(tab_dev_ + ts.off[k_gtile]);
+    d.ltiles = (const GridTileDev*)(tab_dev_ + ts.off[k_ltile]);
+    d.stiles = (const SortTileDev*)(tab_dev_ + ts.off[k_st]);
     d.X = pl_.X;
     d.V = pl_.V;
     d.P[0] = pl_.P0;
@@ -1299,7 +1384,7 @@ void Engine::ensure_committed() {
         VTX_HIP(hipStreamSynchronize(stream_));
         grow_pools(need);
         for (auto& sg : segs_)
-            if (sg && sg->active && !sg->uploaded) upload(*sg);
+            if (sg && sg->active && !sg->uploaded) upload(*sg, stream_);
         tables_dirty_ = true;
     }
     if (tables_dirty_) rebuild_tables();

@@ -13,6 +13,8 @@
 
 namespace vtx {
 
+struct TableSet;   // host images of the launch tables (engine.cpp)
+
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -208,15 +210,28 @@ public:
     std::vector<int32_t> nonfinite_segments();
     uint64_t live_particles() const { return live_particles_; }
     size_t pool_bytes() const;
+    // Upload active segments not yet on the device and rebuild the launch tables, now rather
+    // than at the next frame (returns when the uploads are done).
+    void commit() { ensure_committed(); }
+    // Upload an inactive segment into free pool ranges on a copy stream of its own, so that it
+    // can overlap a frame of the active segments (their ranges are disjoint); returns when the
+    // copies are done. If the pools must grow first, the running frame is waited for (growth
+    // moves every pool). The segment stays inactive; activating it later only rebuilds tables.
+    void stage(int32_t s);
+    // Grow the pools so that segments totalling `particles` particles (grid cloths, default hash
+    // table and neighbour cap) fit without a later growth.
+    void reserve(uint64_t particles);
 
 private:
     void pull_state(Segment& sg);
     void touch(int32_t s);                   // a structural change of segment s
     void invalidate_graph();
     void ensure_committed();
-    void upload(Segment& sg);
+    void upload(Segment& sg, hipStream_t s);
+    bool pools_fit(const Ranges& need);
     void release(Segment& sg);
     void rebuild_tables();
+    void upload_tables(const TableSet& t);
     void grow_pools(const Ranges& need);
     void record_frame(float frame_dt, hipStream_t s);
     void hash_build(const DevState& d, int in, hipStream_t s);
@@ -248,7 +263,9 @@ private:
         ColliderDev* cols = nullptr;
         float* lra = nullptr;
     } pl_;
-    std::vector<void*> table_allocs_;
+    uint8_t* tab_dev_ = nullptr;    // launch tables (segments, tiles, sort tiles), one arena
+    uint8_t* tab_host_ = nullptr;   // pinned staging of the same size
+    size_t tab_cap_ = 0;
     size_t hist_cap_ = 0, part_cap_ = 0, flag_cap_ = 0;
     uint32_t* hist_ = nullptr;
     uint32_t* part_ = nullptr;
@@ -257,6 +274,7 @@ private:
     DevState ds_{};                          // every live segment
     std::vector<DevState> groups_;           // the frame's table sets (1, or 2 on two streams)
     hipStream_t stream2_ = nullptr;
+    hipStream_t cstream_ = nullptr;   // uploads of staged segments
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     std::vector<int32_t> table_seg_;         // table index -> segment id
     int passes_ = 1;

@@ -11,6 +11,12 @@ Server::Server(Engine& e, const vtx_server_config& cfg) : e_(e), cfg_(cfg) {
         limit_ = free_b > cfg_.hbm_reserve_bytes ? free_b - (size_t)cfg_.hbm_reserve_bytes : 0;
     }
     e_.set_use_graph(cfg_.use_graph != 0);
+    e_.reserve(cfg_.reserve_particles);
+}
+
+Server::~Server() {
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
 }
 
 Job& Server::get(int32_t id) {
@@ -57,39 +63,75 @@ void Server::submit(int32_t id) {
 // Step-time admission: the frame time of the live set is predicted as (particle-substeps) x
 // (measured ms per particle-substep). Until a step has been measured, one job is admitted per
 // step. A job always fits an empty server (so one that alone exceeds the budget still runs).
+// The step-time budget bounds the predicted frame of every live job: live particle-substeps x
+// the measured frame time per particle-substep (admission itself is timed apart, admit_ms).
 bool Server::fits(const Job& j, uint32_t admitted_now) const {
     if (resident_ + j.bytes > limit_) return false;
     if (cfg_.max_live_particles && live_particles_ + j.n > cfg_.max_live_particles) return false;
-    if (cfg_.max_live_jobs && live_.size() >= cfg_.max_live_jobs) return false;
-    if (cfg_.step_budget_ms > 0.f && !live_.empty()) {
+    if (cfg_.max_live_jobs && live_.size() + staged_.size() >= cfg_.max_live_jobs) return false;
+    if (cfg_.step_budget_ms > 0.f && !(live_.empty() && staged_.empty())) {
         if (ms_per_work_ == 0.0) return admitted_now == 0;
         if ((double)(live_work_ + j.work) * ms_per_work_ > (double)cfg_.step_budget_ms) return false;
     }
     return true;
 }
 
-void Server::step(float frame_dt, vtx_server_stats* out) {
-    const auto t0 = std::chrono::steady_clock::now();
-    vtx_server_stats st{};
-    // admission, first come first served
+// Admission first come, first served: the queued jobs that fit leave the queue. An empty server
+// activates them for this step's frame; a running server stages them (uploads on the engine's
+// copy stream while this step's frame runs) and they join the live set at the next step.
+uint32_t Server::admit(bool stage, uint32_t before) {
+    uint32_t n = 0;
     while (!queue_.empty()) {
         Job& j = get(queue_.front());
-        if (!fits(j, st.admitted)) break;
+        if (!fits(j, before + n)) break;
         queue_.pop_front();
-        e_.set_active(j.seg, true);
+        if (stage) {
+            e_.stage(j.seg);
+            staged_.push_back(j.id);
+        } else {
+            e_.set_active(j.seg, true);
+            live_.push_back(j.id);
+        }
         j.status = VTX_JOB_RUNNING;
         resident_ += j.bytes;
         live_particles_ += j.n;
         live_work_ += j.work;
-        live_.push_back(j.id);
-        st.admitted++;
+        n++;
     }
+    return n;
+}
+
+void Server::step(float frame_dt, vtx_server_stats* out) {
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    const auto t0 = clk::now();
+    vtx_server_stats st{};
+    for (int32_t id : staged_) {                 // staged last step: uploaded, now live
+        e_.set_active(get(id).seg, true);
+        live_.push_back(id);
+    }
+    staged_.clear();
+    if (live_.empty()) st.admitted += admit(false, 0);
     uint64_t work = 0;
     if (!live_.empty()) {
-        const auto f0 = std::chrono::steady_clock::now();
+        const auto c0 = clk::now();
+        e_.commit();                             // uploads of jobs activated now, launch tables
+        st.admit_ms = ms_since(c0);
+        if (!ev0_) {
+            VTX_HIP(hipEventCreate(&ev0_));
+            VTX_HIP(hipEventCreate(&ev1_));
+        }
+        const hipStream_t es = e_.stream();
+        VTX_HIP(hipEventRecord(ev0_, es));
         e_.simulate(frame_dt);
+        VTX_HIP(hipEventRecord(ev1_, es));
+        const auto s0 = clk::now();
+        st.admitted += admit(true, st.admitted);   // overlaps the frame
+        st.admit_ms += ms_since(s0);
         e_.synchronize();
-        const double frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
+        float frame_ms = 0.f;
+        VTX_HIP(hipEventElapsedTime(&frame_ms, ev0_, ev1_));
+        st.frame_ms = frame_ms;
         for (int32_t id : live_) {
             Job& j = get(id);
             j.done++;
@@ -101,6 +143,7 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
         }
     }
     steps_++;
+    const auto r0 = clk::now();
     std::vector<int32_t> bad;
     if (cfg_.check_every > 0 && steps_ % (uint64_t)cfg_.check_every == 0 && !live_.empty())
         bad = e_.nonfinite_segments();
@@ -126,6 +169,7 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
         }
     }
     live_.swap(keep);
+    st.retire_ms = ms_since(r0);
     st.live_jobs = (uint32_t)live_.size();
     st.queued_jobs = (uint32_t)queue_.size();
     st.live_particles = live_particles_;
@@ -141,7 +185,9 @@ void Server::preempt(int32_t id) {
     Job& j = get(id);
     VTX_REQUIRE(j.status == VTX_JOB_RUNNING, VTX_ESTATE, "only a running job can be preempted");
     e_.park(j.seg);
-    live_.erase(std::find(live_.begin(), live_.end(), id));
+    auto it = std::find(live_.begin(), live_.end(), id);
+    if (it != live_.end()) live_.erase(it);
+    else staged_.erase(std::find(staged_.begin(), staged_.end(), id));   // running jobs are in one of the two
     resident_ -= j.bytes;
     live_particles_ -= j.n;
     live_work_ -= j.work;

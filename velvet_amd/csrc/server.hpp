@@ -25,6 +25,9 @@ struct Job {
 class Server {
 public:
     Server(Engine& e, const vtx_server_config& cfg);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
     int32_t new_job(const vtx_params& p, int32_t frames, int32_t* seg);
     void submit(int32_t job);
     void step(float frame_dt, vtx_server_stats* out);
@@ -35,12 +38,15 @@ public:
 private:
     Job& get(int32_t id);
     bool fits(const Job& j, uint32_t admitted_now) const;
+    uint32_t admit(bool stage, uint32_t before);   // before: jobs admitted earlier in this step
 
     Engine& e_;
     vtx_server_config cfg_;
     std::map<int32_t, Job> jobs_;
     std::deque<int32_t> queue_;
     std::vector<int32_t> live_;
+    std::vector<int32_t> staged_;  // admitted while a frame ran: uploaded, live from the next step
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;   // GPU time of the frame
     int32_t next_id_ = 0;
     size_t limit_ = 0;           // device bytes jobs may occupy
     size_t resident_ = 0;        // estimated bytes of the live jobs
