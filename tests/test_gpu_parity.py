@@ -91,6 +91,24 @@ def test_hash_stage_bit_exact_packed_segments():
     s.close()
 
 
+@pytest.mark.parametrize("shift", [117966.5, -117966.2])
+def test_hash_stage_bit_exact_far_from_origin(shift):
+    """A folded cloth about 2^20 cells from the origin on x (positions with a 1/128 m binary32
+    spacing, cell coordinates beyond 20 bits): cell tables and lists stay bit-exact."""
+    say(f"hash far from origin x+{shift}")
+    c, p = _folded(24, seed=5)
+    off = np.array([shift, 0.0, 0.0], F32)
+    c = S.Cloth(**{**c.__dict__, "x": (c.x + off).astype(F32)})
+    p = (p + off).astype(F32)
+    s = _hash_solver([(c, p)], [0])
+    cell = float(F32(F32(c.diameter) * F32(1.5)))
+    ix = np.floor(p[:, 0] / F32(cell))
+    assert ix.min() < 2 ** 20 <= ix.max() or ix.min() < -(2 ** 20) <= ix.max()
+    ref = oracle.hash32(p, c.x, cell, float(c.diameter), 2 * c.n)
+    _check_hash_equal(s.read_hash(0), ref, c.n)
+    s.close()
+
+
 def test_hash_stage_matches_golden_fixture():
     say("hash golden fixture")
     import os
