@@ -167,6 +167,44 @@ def cpu_baseline(scenes, budget_s):
     return done_ps / spent, spent, frames, cloths
 
 
+def cpu_baseline_threads(scenes, budget_s, workers):
+    """The same restated reference CPU solver on `workers` host threads at once, each taking the
+    workload's cloths in turn (the cloths are independent; the solver calls release the GIL).
+    Per thread: one warm-up frame per cloth, then one timed frame; the aggregate rate is the sum
+    of the threads' rates over their timed frames, which all run concurrently."""
+    import threading
+    import oracle
+    lock = threading.Lock()
+    state = {"next": 0}
+    rates = [0.0] * workers
+    counts = [0] * workers
+    deadline = time.perf_counter() + budget_s
+
+    def work(w):
+        done_ps, spent = 0, 0.0
+        while time.perf_counter() < deadline:
+            with lock:
+                k = state["next"]
+                state["next"] += 1
+            sc = scenes[k % len(scenes)]
+            cpu = oracle.CpuSolver(sc)
+            cpu.simulate()
+            t0 = time.perf_counter()
+            cpu.simulate()
+            spent += time.perf_counter() - t0
+            done_ps += sc.n * sc.params.num_substeps
+            counts[w] += 1
+            del cpu
+        rates[w] = done_ps / spent if spent else 0.0
+
+    threads = [threading.Thread(target=work, args=(w,)) for w in range(workers)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    return sum(rates), sum(counts)
+
+
 def parse_tuning(text):
     """'k=v,k=v' -> {k: int(v)}"""
     return {k: int(v) for k, v in (kv.split("=") for kv in text.split(",") if kv)}
@@ -354,7 +392,7 @@ def main():
                                      args.serve_budget_ms, tuning=parse_tuning(args.serve_tuning)))
             log(f"  {serving[-1]}")
 
-    cpu = None
+    cpu = cpu_threads = None
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_seconds:.0f} s budget)")
         rate, spent, frames, cloths = cpu_baseline(scenes, args.cpu_seconds)
@@ -363,6 +401,13 @@ def main():
                          f"SpatialHashCPU), single thread, 1 timed frame each of the first {cloths} "
                          f"cloths of the workload after a warm-up frame, {spent:.1f} s",
                "host_cpu": _cpu_model()}
+        workers = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {workers} threads ({args.cpu_seconds:.0f} s budget)")
+        mrate, mcloths = cpu_baseline_threads(scenes, args.cpu_seconds, workers)
+        cpu_threads = {"value": mrate, "unit": "particle-substeps/s", "cores": workers, "kind": "port",
+                       "sample": f"the same solver on {workers} host threads at once, one timed frame "
+                                 f"per cloth after a warm-up frame, {mcloths} cloths, sum of the "
+                                 f"threads' rates"}
 
     line = {"metric": METRIC, "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -375,7 +420,8 @@ def main():
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "frames": len(lat)},
             "kernels_ms": {k: v for k, v in kt.items()},
-            "serving": serving, "roofline": roofline, "cpu_baseline": cpu}
+            "serving": serving, "roofline": roofline, "cpu_baseline": cpu,
+            "cpu_baseline_threads": cpu_threads}
     print(json.dumps(line), flush=True)
 
 
