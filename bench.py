@@ -71,6 +71,8 @@ def workload_scenes(name):
         return [S.config2()], "256x256 cloth, 256-slot LRA edge, sphere + plane, 2^18 cells, cell = d, 5x10"
     if name == "config3":
         return [S.config3()], "1024x1024 cloth, 3 seeded spheres, 2^21 cells, 4x8"
+    if name == "serve64":
+        return [sc for sc, _ in serve_pool(0)], "the 64 serving job descriptions as one batch, 5x10"
     raise SystemExit(f"unknown workload {name}")
 
 
