@@ -154,8 +154,8 @@ typedef struct vtx_tuning {
     float frame_ms;            /* vtx_autotune: the chosen variant's frame time (0 from the cache) */
     int32_t gather_block;      /* threads per workgroup of the neighbour search and contact pass:
                                   64, 128 or 256 (each 256-particle tile split across workgroups) */
-    int32_t streams;           /* 2: the segments split into two particle-balanced groups whose
-                                  frames run concurrently on two streams; 1: one stream */
+    int32_t streams;           /* k = 1..4: the segments split into k particle-balanced groups
+                                  whose frames run concurrently on k streams */
     int32_t lra_split;         /* 1: grid cloths with 32 or more dense attachment slots run their
                                   Jacobi pass with four threads per particle; 0: one thread */
 } vtx_tuning;

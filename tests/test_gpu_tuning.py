@@ -52,9 +52,9 @@ def _run(scenes, frames, **tuning):
 # every value of every knob on its own, plus mixed combinations
 VARIANTS = ([dict(use_graph=g) for g in (0, 1)] + [dict(neighbor_order=o) for o in (0, 1, 2)]
             + [dict(lra_batch=b) for b in (4, 16, 64)] + [dict(xcd_remap=r) for r in (0, 1)]
-            + [dict(gather_block=b) for b in (64, 128, 256)] + [dict(streams=k) for k in (1, 2)]
+            + [dict(gather_block=b) for b in (64, 128, 256)] + [dict(streams=k) for k in (1, 2, 3, 4)]
             + [dict(use_graph=1, neighbor_order=2, lra_batch=4, xcd_remap=0, gather_block=64, streams=2),
-               dict(use_graph=0, neighbor_order=0, lra_batch=64, xcd_remap=1, gather_block=128, streams=2),
+               dict(use_graph=0, neighbor_order=0, lra_batch=64, xcd_remap=1, gather_block=128, streams=4),
                dict(use_graph=1, neighbor_order=1, lra_batch=16, xcd_remap=0, gather_block=64, streams=1)])
 
 
@@ -88,7 +88,7 @@ def test_lra_split_in_a_mixed_batch():
     lra = S.Scene(small[0].params, [big], small[0].colliders, 3, "lra40")
     scenes = [small[0], lra, small[1], small[2]]
     alone = [_run([sc], 3)[0] for sc in scenes]
-    for v in (dict(), dict(lra_split=0), dict(streams=2), dict(streams=2, lra_split=0, lra_batch=4)):
+    for v in (dict(), dict(lra_split=0), dict(streams=2), dict(streams=3, lra_split=0, lra_batch=4)):
         say(f"  {v}")
         got = _run(scenes, 3, **v)
         for k in range(len(scenes)):
@@ -138,6 +138,6 @@ def test_tuning_rejects_bad_values():
         with pytest.raises(Exception):
             s.set_tuning(gather_block=96)
         with pytest.raises(Exception):
-            s.set_tuning(streams=3)
+            s.set_tuning(streams=5)
         s.set_tuning(lra_split=0)
         assert s.tuning()["lra_split"] == 0

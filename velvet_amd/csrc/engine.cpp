@@ -272,16 +272,16 @@ Engine::~Engine() {
     if (hist_) (void)hipFree(hist_);
     if (part_) (void)hipFree(part_);
     if (flags_) (void)hipFree(flags_);
-    if (stream2_) {
-        (void)hipStreamSynchronize(stream2_);
-        (void)hipStreamDestroy(stream2_);
+    for (hipStream_t x : xstreams_) {
+        (void)hipStreamSynchronize(x);
+        (void)hipStreamDestroy(x);
     }
     if (cstream_) {
         (void)hipStreamSynchronize(cstream_);
         (void)hipStreamDestroy(cstream_);
     }
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
-    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    for (hipEvent_t e : ev_join_) (void)hipEventDestroy(e);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1188,11 +1188,12 @@ void Engine::rebuild_tables() {
             total_n += sp->n();
             nlive++;
         }
-    const bool split = tun_.streams == 2 && nlive >= 2 && !timing_;
+    // G groups of about total/G particles each (a prefix of the segment table per group); timing
+    // mode keeps one group so that per-launch figures cover the whole workload
+    const uint32_t G = timing_ ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)tun_.streams, nlive));
+    struct Cut { size_t t, i, g, l, s; uint32_t h, p; };
+    std::vector<Cut> cuts;                     // where groups 1..G-1 begin
     uint64_t cum = 0;
-    bool in_second = false;
-    size_t t_split = 0, i_split = 0, g_split = 0, l_split = 0, s_split = 0;
-    uint32_t h_split = 0, p_split = 0;
     for (size_t si = 0; si < segs_.size(); si++) {
         Segment* sgp = segs_[si].get();
         if (!sgp || !sgp->active || !sgp->uploaded || sgp->n() == 0) continue;
@@ -1206,16 +1207,10 @@ void Engine::rebuild_tables() {
         }
         VTX_REQUIRE(p.num_substeps == substeps_ && p.num_iterations == iterations_ && p.interleaved_hash == interleave_,
                     VTX_EINVAL, "active segments of one solver must share substeps, iterations and interleave");
-        if (split && !in_second && cum > 0 && 2 * cum >= total_n) {
-            in_second = true;
-            t_split = htile.size();
-            i_split = hitile.size();
-            g_split = hgtile.size();
-            l_split = hltile.size();
-            s_split = hst.size();
-            h_split = hbase;
-            p_split = (uint32_t)cum;
-            compact = 0;
+        if (cuts.size() + 1 < G && cum > 0 && cum * G >= total_n * (cuts.size() + 1)) {
+            cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hbase,
+                               (uint32_t)cum});
+            compact = 0;                       // each group sorts on its own
         }
         cum += sg.n();
         const uint32_t ti = (uint32_t)hseg.size();
@@ -1263,7 +1258,7 @@ void Engine::rebuild_tables() {
     }
     passes_ = std::max(1, (int)((maxbits + 7) / 8));
     VTX_REQUIRE((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK <= 65536u, VTX_ECAPACITY, "histogram too large");
-    const size_t npart = 2 * ((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1);   // room for two groups
+    const size_t npart = VTX_MAX_STREAMS * ((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK + 1);   // a share per group
     if (hbase > hist_cap_) {
         if (hist_) (void)hipFree(hist_);
         hist_cap_ = hbase + hbase / 2;
@@ -1332,36 +1327,38 @@ void Engine::rebuild_tables() {
     d.hbase0 = 0;
     ds_ = d;
     groups_.clear();
-    if (in_second) {
-        DevState g0 = d, g1 = d;
-        g0.ntiles = (uint32_t)t_split;
-        g0.nitiles = (uint32_t)i_split;
-        g0.ngtiles = (uint32_t)g_split;
-        g0.nltiles = (uint32_t)l_split;
-        g0.nstiles = (uint32_t)s_split;
-        g0.nhist = h_split;
-        g0.nparticles = p_split;
-        g1.tiles = d.tiles + t_split;
-        g1.itiles = d.itiles + i_split;
-        g1.gtiles = d.gtiles + g_split;
-        g1.ltiles = d.ltiles + l_split;
-        g1.stiles = d.stiles + s_split;
-        g1.ntiles = d.ntiles - (uint32_t)t_split;
-        g1.nitiles = d.nitiles - (uint32_t)i_split;
-        g1.ngtiles = d.ngtiles - (uint32_t)g_split;
-        g1.nltiles = d.nltiles - (uint32_t)l_split;
-        g1.nstiles = d.nstiles - (uint32_t)s_split;
-        g1.hbase0 = h_split;
-        g1.nhist = d.nhist - h_split;
-        g1.nparticles = d.nparticles - p_split;
-        g1.part = part_ + part_cap_ / 2;
-        groups_.push_back(g0);
-        groups_.push_back(g1);
-        if (!stream2_) VTX_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hbase, d.nparticles});
+    Cut lo{0, 0, 0, 0, 0, 0, 0};
+    for (size_t k = 0; k < cuts.size(); k++) {
+        const Cut& hi = cuts[k];
+        DevState g = d;
+        g.tiles = d.tiles + lo.t;
+        g.itiles = d.itiles + lo.i;
+        g.gtiles = d.gtiles + lo.g;
+        g.ltiles = d.ltiles + lo.l;
+        g.stiles = d.stiles + lo.s;
+        g.ntiles = (uint32_t)(hi.t - lo.t);
+        g.nitiles = (uint32_t)(hi.i - lo.i);
+        g.ngtiles = (uint32_t)(hi.g - lo.g);
+        g.nltiles = (uint32_t)(hi.l - lo.l);
+        g.nstiles = (uint32_t)(hi.s - lo.s);
+        g.hbase0 = lo.h;
+        g.nhist = hi.h - lo.h;
+        g.nparticles = hi.p - lo.p;
+        g.part = part_ + k * (part_cap_ / VTX_MAX_STREAMS);
+        groups_.push_back(g);
+        lo = hi;
+    }
+    if (groups_.size() > 1) {
+        while (xstreams_.size() + 1 < groups_.size()) {
+            hipStream_t x = nullptr;
+            hipEvent_t e = nullptr;
+            VTX_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+            xstreams_.push_back(x);
+            VTX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev_join_.push_back(e);
+        }
         if (!ev_fork_) VTX_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-        if (!ev_join_) VTX_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
-    } else {
-        groups_.push_back(d);
     }
     tables_dirty_ = false;
 }
@@ -1484,20 +1481,22 @@ void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s) {
     stage_end(s);
 }
 
-// One frame: with two table sets the second runs on its own stream, forked from and joined
-// back into the solver's stream (also inside a graph capture), so the segments of the two
-// groups advance concurrently and every later operation on the solver's stream sees both.
+// One frame: with several table sets, group k > 0 runs on stream k of its own, forked from and
+// joined back into the solver's stream (also inside a graph capture), so the groups' segments
+// advance concurrently and every later operation on the solver's stream sees all of them.
 void Engine::record_frame(float frame_dt, hipStream_t s) {
     if (groups_.size() < 2) {
         record_group(groups_.empty() ? ds_ : groups_[0], frame_dt, s);
         return;
     }
     VTX_HIP(hipEventRecord(ev_fork_, s));
-    VTX_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+    for (size_t k = 1; k < groups_.size(); k++) VTX_HIP(hipStreamWaitEvent(xstreams_[k - 1], ev_fork_, 0));
     record_group(groups_[0], frame_dt, s);
-    record_group(groups_[1], frame_dt, stream2_);
-    VTX_HIP(hipEventRecord(ev_join_, stream2_));
-    VTX_HIP(hipStreamWaitEvent(s, ev_join_, 0));
+    for (size_t k = 1; k < groups_.size(); k++) {
+        record_group(groups_[k], frame_dt, xstreams_[k - 1]);
+        VTX_HIP(hipEventRecord(ev_join_[k - 1], xstreams_[k - 1]));
+    }
+    for (size_t k = 1; k < groups_.size(); k++) VTX_HIP(hipStreamWaitEvent(s, ev_join_[k - 1], 0));
 }
 
 void Engine::simulate(float frame_dt) {
@@ -1554,7 +1553,7 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
     VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
                 "gather_block must be 64, 128 or 256");
-    VTX_REQUIRE(t.streams == 1 || t.streams == 2, VTX_EINVAL, "streams must be 1 or 2");
+    VTX_REQUIRE(t.streams >= 1 && t.streams <= VTX_MAX_STREAMS, VTX_EINVAL, "streams must be 1 to 4");
     if (t.streams != tun_.streams || t.lra_split != tun_.lra_split) tables_dirty_ = true;
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
@@ -1657,7 +1656,7 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
             if (knob == 4 && !need_collide_) continue;
             if (knob == 5 && table_seg_.size() < 2) continue;
             if (knob == 6 && !any_lra_split_) continue;
-            const int nval = knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
+            const int nval = knob == 5 ? VTX_MAX_STREAMS : knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
                 Tuning t = base;

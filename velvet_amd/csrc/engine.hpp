@@ -122,8 +122,8 @@ struct Tuning {
     int lra_batch = 16;       // dense attachment distances loaded per batch: 4, 16 or 64
     bool remap = true;        // XCD-aware workgroup -> tile remap
     int gather_block = 256;   // threads per workgroup of the neighbour search and contact pass
-    int streams = 1;          // 2: segments split into two particle-balanced groups whose frames
-                              // run on two streams, overlapping latency- and VALU-bound stages
+    int streams = 1;          // k > 1: segments split into k particle-balanced groups whose frames
+                              // run on k streams, overlapping latency- and VALU-bound stages
     bool lra_split = true;    // grid segments with >= VTX_LRA_SPLIT_MIN dense slots: four threads
                               // per particle in the Jacobi pass
 };
@@ -272,10 +272,11 @@ private:
     uint32_t* flags_ = nullptr;
 
     DevState ds_{};                          // every live segment
-    std::vector<DevState> groups_;           // the frame's table sets (1, or 2 on two streams)
-    hipStream_t stream2_ = nullptr;
+    std::vector<DevState> groups_;           // the frame's table sets, one per stream
+    std::vector<hipStream_t> xstreams_;      // streams of groups 1.. (created on demand)
     hipStream_t cstream_ = nullptr;   // uploads of staged segments
-    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr;
+    std::vector<hipEvent_t> ev_join_;        // one per extra stream
     std::vector<int32_t> table_seg_;         // table index -> segment id
     int passes_ = 1;
     int sorted_src_ = 0;

@@ -41,6 +41,8 @@
 // lanes one part each (stencil segments with at least VTX_LRA_SPLIT_MIN dense slots), with the
 // same result.
 #define VTX_LRA_PARTS 4
+// Most streams a frame's segment groups run on (vtx_tuning.streams).
+#define VTX_MAX_STREAMS 4
 #define VTX_LRA_SPLIT_MIN 32u
 
 // 2-D grid tiles of the stencil path: VTX_GT x VTX_GT particles, halo of 2 rows/cols.
