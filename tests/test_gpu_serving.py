@@ -152,6 +152,30 @@ def test_preempt_resume_equals_alone():
             sv.preempt(ids[0])          # done jobs cannot be preempted
 
 
+def test_preempt_staged_job():
+    """A job admitted while another runs is uploaded during that frame and joins at the next step;
+    preempted before it has run a frame, it goes back to the queue with 0 frames done and still
+    ends bit-identical to its solo run."""
+    say("serving preemption of a staged job")
+    js = jobs(2)
+    with Server() as sv:
+        a = sv.submit(*js[0])
+        sv.step()                                   # empty server: a starts at once
+        assert sv.status(a) == (RUNNING, 1)
+        b = sv.submit(*js[1])
+        st = sv.step()                              # b is admitted during a's frame
+        assert st.admitted == 1 and st.live_jobs == 1 and sv.status(b) == (RUNNING, 0)
+        sv.preempt(b)
+        assert sv.status(b) == (QUEUED, 0)
+        for _ in range(60):
+            sv.step()
+            if all(sv.status(j)[0] == DONE for j in (a, b)):
+                break
+        for (sc, f), j in zip(js, (a, b)):
+            assert sv.status(j) == (DONE, f)
+            assert np.array_equal(sv.result(j), alone(sc, f))
+
+
 def test_nonfinite_job_is_evicted():
     say("serving failure detection")
     js = jobs(3)
