@@ -913,7 +913,7 @@ void Engine::grow_pools(const Ranges& need) {
     };
     grow(ap_, need.n_p, [&](size_t o, size_t c) {
         regrow(pl_.X, o, c); regrow(pl_.V, o, c); regrow(pl_.P0, o, c); regrow(pl_.P1, o, c);
-        regrow(pl_.X0, o, c); regrow(pl_.N, o, c); regrow(pl_.sP, o, c); regrow(pl_.sX0, o, c);
+        regrow(pl_.X0, o, c); regrow(pl_.N, o, c); regrow(pl_.sPX, 2 * o, 2 * c);
         regrow(pl_.rest4, o, c); regrow(pl_.brest4, o, c); regrow(pl_.k0, o, c); regrow(pl_.k1, o, c);
         regrow(pl_.v0, o, c); regrow(pl_.v1, o, c); regrow(pl_.gflags, o, c);
     });
@@ -967,7 +967,7 @@ void Engine::reserve(uint64_t particles) {
 }
 
 void Engine::free_pools() {
-    void* ptrs[] = {pl_.X, pl_.V, pl_.P0, pl_.P1, pl_.X0, pl_.N, pl_.sP, pl_.sX0, pl_.rest4, pl_.brest4,
+    void* ptrs[] = {pl_.X, pl_.V, pl_.P0, pl_.P1, pl_.X0, pl_.N, pl_.sPX, pl_.rest4, pl_.brest4,
                     pl_.k0, pl_.k1, pl_.v0, pl_.v1, pl_.gflags, pl_.cells, pl_.nbr, pl_.st, pl_.at, pl_.bd,
                     pl_.tr, pl_.brec, pl_.trec, pl_.brest, pl_.slots, pl_.cols, pl_.lra};
     for (void* p : ptrs)
@@ -1300,8 +1300,7 @@ void Engine::rebuild_tables() {
     d.P[1] = pl_.P1;
     d.X0 = pl_.X0;
     d.Nrm = pl_.N;
-    d.sP = pl_.sP;
-    d.sX0 = pl_.sX0;
+    d.sPX = pl_.sPX;
     d.rest4 = pl_.rest4;
     d.brest4 = pl_.brest4;
     d.gflags = pl_.gflags;

@@ -251,7 +251,7 @@ private:
     RangeAlloc ap_, acell_, anbr_, ast_, aat_, abd_, atr_, abrec_, atrec_, aslot_, acol_, alra_;
     struct Pools {
         float4 *X = nullptr, *V = nullptr, *P0 = nullptr, *P1 = nullptr, *X0 = nullptr, *N = nullptr,
-               *sP = nullptr, *sX0 = nullptr, *rest4 = nullptr, *brest4 = nullptr;
+               *sPX = nullptr, *rest4 = nullptr, *brest4 = nullptr;
         uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *gflags = nullptr;
         uint2* cells = nullptr;
         uint32_t* nbr = nullptr;

@@ -235,8 +235,8 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
     // local index (the inverse mass is not needed there)
     const uint32_t id = d.vals[src][g];
     const float4 q = d.P[in][sg.pbase + id];
-    d.sP[g] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
-    d.sX0[g] = d.X0[sg.pbase + id];
+    d.sPX[2 * (size_t)g] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
+    d.sPX[2 * (size_t)g + 1] = d.X0[sg.pbase + id];
 }
 
 // CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130). Cells are visited x-, y-, z-nested;
@@ -257,10 +257,11 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // next candidate's position and initial position loaded while the current one is tested.
 #define VTX_NB_CACHE 16
 
-// Element i < 2^25 of a float4 array whose base is wave-uniform: a 32-bit byte offset lets the
+// The float4 at record i < 2^25 of the 32-B sorted records from a wave-uniform base (the
+// position half or, one float4 on, the initial-position half): a 32-bit byte offset lets the
 // load use the scalar-base + 32-bit vector-offset form.
 __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
-    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (size_t)((i & 0x1FFFFFFu) << 4));
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (size_t)((i & 0x1FFFFFFu) << 5));
 }
 
 template <int ORDER, int BS>
@@ -286,12 +287,12 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const SegDev sg = d.segs[seg];
     if (!(sg.flags & VTX_SEG_SELFCOL)) return;
     const uint32_t n = sg.n;
-    const float4* __restrict__ sP = d.sP + sg.pbase;
-    const float4* __restrict__ sX0 = d.sX0 + sg.pbase;
+    const float4* __restrict__ sP = d.sPX + 2 * (size_t)sg.pbase;   // position of sorted i at 2i, initial at 2i+1
+    const float4* __restrict__ sX0 = sP + 1;
     const uint2* __restrict__ cells = d.cells + sg.tbase;
     const bool sorted = ORDER == NBR_SORTED;
-    const float4 p = sorted ? sP[i] : d.P[in][sg.pbase + i];
-    const float4 o = sorted ? sX0[i] : d.X0[sg.pbase + i];
+    const float4 p = sorted ? sP[2 * i] : d.P[in][sg.pbase + i];
+    const float4 o = sorted ? sX0[2 * i] : d.X0[sg.pbase + i];
     const uint32_t id = sorted ? __float_as_uint(p.w) : i;
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
     const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);

@@ -131,8 +131,8 @@ struct DevState {
     uint32_t* hist;           // per (segment, digit, sort tile) counts, scanned in place
     uint32_t* part;           // scan partials
     uint2* cells;             // per hash cell: (start, end) in sorted order, start ~0u = empty
-    float4* sP;               // predicted positions in sorted order, w = local particle index
-    float4* sX0;              // initial positions gathered into sorted order
+    float4* sPX;              // per sorted position, 32 B: the predicted position with w = local
+                              // particle index, then the initial position (one cache line holds both)
     uint32_t* nbr;            // neighbour tables, column-major per segment
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
     const GridTileDev* gtiles;
