@@ -158,6 +158,9 @@ typedef struct vtx_tuning {
                                   whose frames run concurrently on k streams */
     int32_t lra_split;         /* 1: grid cloths with 32 or more dense attachment slots run their
                                   Jacobi pass with four threads per particle; 0: one thread */
+    int32_t hash_lds;          /* 1: self-colliding segments of at most 16384 particles (table of
+                                  at most 2^16 cells) build their spatial hash in one workgroup;
+                                  0: every segment takes the device-wide sort */
 } vtx_tuning;
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);   /* vtx_set_tuning ignores frame_ms */
 int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);

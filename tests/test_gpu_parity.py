@@ -91,6 +91,28 @@ def test_hash_stage_bit_exact_packed_segments():
     s.close()
 
 
+@pytest.mark.parametrize("hash_lds", [1, 0])
+def test_hash_stage_bit_exact_mixed_paths(hash_lds):
+    """Segments hashed in one workgroup each (at most 16384 particles, at most 2^16 cells) packed
+    with segments that take the device-wide sort (131 x 131 = 17161 particles; a small cloth with a
+    2^17-cell table): every segment's tables and lists match the oracle, with the small-segment
+    path on or off."""
+    say(f"hash mixed paths hash_lds={hash_lds}")
+    items = [_folded(16, seed=1), _folded(131, seed=2), _folded(40, seed=3), _folded(20, seed=4)]
+    sizes = [0, 0, 1 << 17, 97]
+    s = _hash_solver(items, sizes)
+    s.set_tuning(hash_lds=hash_lds)
+    for k, (c, p) in enumerate(items):
+        s.write_predicted(p, seg=k)
+    s.run_hash()
+    for k, (c, p) in enumerate(items):
+        Mx = sizes[k] if sizes[k] else 2 * c.n
+        cell = float(F32(F32(c.diameter) * F32(1.5)))
+        ref = oracle.hash32(p, c.x, cell, float(c.diameter), Mx)
+        _check_hash_equal(s.read_hash(k), ref, c.n)
+    s.close()
+
+
 @pytest.mark.parametrize("shift", [117966.5, -117966.2])
 def test_hash_stage_bit_exact_far_from_origin(shift):
     """A folded cloth about 2^20 cells from the origin on x (positions with a 1/128 m binary32

@@ -52,10 +52,10 @@ def _run(scenes, frames, **tuning):
 # every value of every knob on its own, plus mixed combinations
 VARIANTS = ([dict(use_graph=g) for g in (0, 1)] + [dict(neighbor_order=o) for o in (0, 1, 2)]
             + [dict(lra_batch=b) for b in (4, 16, 64)] + [dict(xcd_remap=r) for r in (0, 1)]
-            + [dict(gather_block=b) for b in (64, 128, 256)] + [dict(streams=k) for k in (1, 2, 3, 4)]
+            + [dict(gather_block=b) for b in (64, 128, 256)] + [dict(streams=k) for k in (1, 2, 3, 4)] + [dict(hash_lds=h) for h in (0, 1)]
             + [dict(use_graph=1, neighbor_order=2, lra_batch=4, xcd_remap=0, gather_block=64, streams=2),
                dict(use_graph=0, neighbor_order=0, lra_batch=64, xcd_remap=1, gather_block=128, streams=4),
-               dict(use_graph=1, neighbor_order=1, lra_batch=16, xcd_remap=0, gather_block=64, streams=1)])
+               dict(use_graph=1, neighbor_order=1, lra_batch=16, xcd_remap=0, gather_block=64, streams=1, hash_lds=0)])
 
 
 @pytest.mark.parametrize("which", ["grid batch", "general"])

@@ -40,7 +40,8 @@ class VtxServerConfig(C.Structure):
 
 class VtxTuning(C.Structure):
     _fields_ = [("use_graph", i32), ("neighbor_order", i32), ("lra_batch", i32), ("xcd_remap", i32),
-                ("frame_ms", f32), ("gather_block", i32), ("streams", i32), ("lra_split", i32)]
+                ("frame_ms", f32), ("gather_block", i32), ("streams", i32), ("lra_split", i32),
+                ("hash_lds", i32)]
 
 
 class VtxServerStats(C.Structure):

@@ -213,6 +213,7 @@ static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
     out->gather_block = t.gather_block;
     out->streams = t.streams;
     out->lra_split = t.lra_split ? 1 : 0;
+    out->hash_lds = t.hash_lds ? 1 : 0;
 }
 
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
@@ -231,6 +232,7 @@ int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
     k.gather_block = t->gather_block;
     k.streams = t->streams;
     k.lra_split = t->lra_split != 0;
+    k.hash_lds = t->hash_lds != 0;
     s->engine.set_tuning(k);
     API_END
 }

@@ -1170,11 +1170,13 @@ void Engine::rebuild_tables() {
     std::vector<SegDev> hseg;
     std::vector<TileDev> htile, hitile;
     std::vector<GridTileDev> hgtile, hltile;
+    std::vector<uint32_t> hsmall;                // table indices of segments hashed in LDS
     std::vector<SortTileDev> hst;
     table_seg_.clear();
     any_selfcol_ = false;
     any_lra_dense_ = false;
     any_lra_split_ = false;
+    any_hash_lds_ = false;
     need_collide_ = false;
     live_particles_ = 0;
     uint32_t maxbits = 0, hbase = 0, compact = 0;
@@ -1191,7 +1193,7 @@ void Engine::rebuild_tables() {
     // G groups of about total/G particles each (a prefix of the segment table per group); timing
     // mode keeps one group so that per-launch figures cover the whole workload
     const uint32_t G = timing_ ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)tun_.streams, nlive));
-    struct Cut { size_t t, i, g, l, s; uint32_t h, p; };
+    struct Cut { size_t t, i, g, l, s, m; uint32_t h, p; };
     std::vector<Cut> cuts;                     // where groups 1..G-1 begin
     uint64_t cum = 0;
     for (size_t si = 0; si < segs_.size(); si++) {
@@ -1208,19 +1210,26 @@ void Engine::rebuild_tables() {
         VTX_REQUIRE(p.num_substeps == substeps_ && p.num_iterations == iterations_ && p.interleaved_hash == interleave_,
                     VTX_EINVAL, "active segments of one solver must share substeps, iterations and interleave");
         if (cuts.size() + 1 < G && cum > 0 && cum * G >= total_n * (cuts.size() + 1)) {
-            cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hbase,
-                               (uint32_t)cum});
+            cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hsmall.size(),
+                               hbase, (uint32_t)cum});
             compact = 0;                       // each group sorts on its own
         }
         cum += sg.n();
         const uint32_t ti = (uint32_t)hseg.size();
         table_seg_.push_back((int32_t)si);
+        const bool hlds = (sg.dev.flags & VTX_SEG_SELFCOL) && sg.dev.n <= VTX_HASH_LDS_N && sg.dev.M <= (1u << 16);
+        any_hash_lds_ = any_hash_lds_ || hlds;
+        const bool in_lds = hlds && tun_.hash_lds;
         hseg.push_back(sg.dev);
+        if (in_lds) {
+            hseg.back().flags |= VTX_SEG_HASH_LDS;
+            hsmall.push_back(ti);
+        }
         const SegDev& d = sg.dev;
         live_particles_ += d.n;
         if (d.flags & VTX_SEG_SELFCOL) {
             any_selfcol_ = true;
-            maxbits = std::max(maxbits, std::max(1u, ceil_log2(d.M)));
+            if (!in_lds) maxbits = std::max(maxbits, std::max(1u, ceil_log2(d.M)));
         }
         if ((d.flags & VTX_SEG_SELFCOL) || d.ccount) need_collide_ = true;
         if (d.flags & VTX_SEG_LRA_DENSE) any_lra_dense_ = true;
@@ -1241,7 +1250,7 @@ void Engine::rebuild_tables() {
             gt.at_off += (uint32_t)sg.rg.at;
             (lsplit && tun_.lra_split ? hltile : hgtile).push_back(gt);
         }
-        const uint32_t nst = (d.n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
+        const uint32_t nst = in_lds ? 0u : (d.n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
         for (uint32_t k = 0; k < nst; k++) {
             SortTileDev st{};
             st.seg = ti;
@@ -1254,7 +1263,7 @@ void Engine::rebuild_tables() {
             hst.push_back(st);
         }
         hbase += 256u * nst;
-        compact += d.n;
+        if (!in_lds) compact += d.n;
     }
     passes_ = std::max(1, (int)((maxbits + 7) / 8));
     VTX_REQUIRE((hbase + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK <= 65536u, VTX_ECAPACITY, "histogram too large");
@@ -1281,12 +1290,21 @@ void Engine::rebuild_tables() {
     d.nitiles = (uint32_t)hitile.size();
     d.ngtiles = (uint32_t)hgtile.size();
     d.nltiles = (uint32_t)hltile.size();
+    d.nhsegs = (uint32_t)hsmall.size();
     d.nstiles = (uint32_t)hst.size();
     d.nhist = hbase;
     d.ncells = (uint32_t)acell_.cap;
+    {   // largest small segments first, within each group's range, so the longest builds start first
+        std::vector<size_t> edge{0};
+        for (const Cut& c : cuts) edge.push_back(c.m);
+        edge.push_back(hsmall.size());
+        for (size_t k = 0; k + 1 < edge.size(); k++)
+            std::stable_sort(hsmall.begin() + (ptrdiff_t)edge[k], hsmall.begin() + (ptrdiff_t)edge[k + 1],
+                             [&](uint32_t a, uint32_t b) { return hseg[a].n > hseg[b].n; });
+    }
     TableSet ts;
     const size_t k_seg = ts.add(hseg), k_tile = ts.add(htile), k_itile = ts.add(hitile), k_gtile = ts.add(hgtile),
-                 k_ltile = ts.add(hltile), k_st = ts.add(hst);
+                 k_ltile = ts.add(hltile), k_st = ts.add(hst), k_hs = ts.add(hsmall);
     upload_tables(ts);
     d.segs = (const SegDev*)(tab_dev_ + ts.off[k_seg]);
     d.tiles = (const TileDev*)(tab_dev_ + ts.off[k_tile]);
@@ -1294,6 +1312,7 @@ void Engine::rebuild_tables() {
     d.gtiles = (const GridTileDev*)(tab_dev_ + ts.off[k_gtile]);
     d.ltiles = (const GridTileDev*)(tab_dev_ + ts.off[k_ltile]);
     d.stiles = (const SortTileDev*)(tab_dev_ + ts.off[k_st]);
+    d.hsegs = (const uint32_t*)(tab_dev_ + ts.off[k_hs]);
     d.X = pl_.X;
     d.V = pl_.V;
     d.P[0] = pl_.P0;
@@ -1326,8 +1345,9 @@ void Engine::rebuild_tables() {
     d.hbase0 = 0;
     ds_ = d;
     groups_.clear();
-    cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hbase, d.nparticles});
-    Cut lo{0, 0, 0, 0, 0, 0, 0};
+    cuts.push_back(Cut{htile.size(), hitile.size(), hgtile.size(), hltile.size(), hst.size(), hsmall.size(), hbase,
+                       d.nparticles});
+    Cut lo{0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t k = 0; k < cuts.size(); k++) {
         const Cut& hi = cuts[k];
         DevState g = d;
@@ -1336,6 +1356,8 @@ void Engine::rebuild_tables() {
         g.gtiles = d.gtiles + lo.g;
         g.ltiles = d.ltiles + lo.l;
         g.stiles = d.stiles + lo.s;
+        g.hsegs = d.hsegs + lo.m;
+        g.nhsegs = (uint32_t)(hi.m - lo.m);
         g.ntiles = (uint32_t)(hi.t - lo.t);
         g.nitiles = (uint32_t)(hi.i - lo.i);
         g.ngtiles = (uint32_t)(hi.g - lo.g);
@@ -1423,20 +1445,28 @@ void Engine::stage_end(hipStream_t s) {
 
 // SpatialHashGPU::Hash -> HashObjects (Velvet/SpatialHashGPU.hpp:41-52, Velvet/SpatialHashGPU.cu:159-196).
 void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
-    stage_begin("k_hash_keys", s);
-    VTX_HIP(launch_hash_keys(d, in, s));
+    // small segments: the whole build in one workgroup each, into the buffer the device-wide
+    // sort of the other segments ends in
+    stage_begin("k_hash_small", s);
+    VTX_HIP(launch_hash_small(d, in, passes_ & 1, s));
     stage_end(s);
-    stage_begin("hash_sort", s);
-    int src = 0;
-    for (int pass = 0; pass < passes_; pass++) {
-        VTX_HIP(launch_sort_pass(d, src, 8 * pass, s));
-        src ^= 1;
+    int src = passes_ & 1;
+    if (d.nstiles) {   // segments on the device-wide path (every segment has sort tiles but those above)
+        stage_begin("k_hash_keys", s);
+        VTX_HIP(launch_hash_keys(d, in, s));
+        stage_end(s);
+        stage_begin("hash_sort", s);
+        src = 0;
+        for (int pass = 0; pass < passes_; pass++) {
+            VTX_HIP(launch_sort_pass(d, src, 8 * pass, s));
+            src ^= 1;
+        }
+        stage_end(s);
+        stage_begin("hash_cells", s);
+        VTX_HIP(launch_clear_cells(d, s));
+        VTX_HIP(launch_cell_bounds(d, in, src, s));
+        stage_end(s);
     }
-    stage_end(s);
-    stage_begin("hash_cells", s);
-    VTX_HIP(launch_clear_cells(d, s));
-    VTX_HIP(launch_cell_bounds(d, in, src, s));
-    stage_end(s);
     stage_begin("k_neighbors", s);
     VTX_HIP(launch_neighbors(d, in, tun_.nbr_order, tun_.gather_block, s));
     stage_end(s);
@@ -1553,7 +1583,7 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
                 "gather_block must be 64, 128 or 256");
     VTX_REQUIRE(t.streams >= 1 && t.streams <= VTX_MAX_STREAMS, VTX_EINVAL, "streams must be 1 to 4");
-    if (t.streams != tun_.streams || t.lra_split != tun_.lra_split) tables_dirty_ = true;
+    if (t.streams != tun_.streams || t.lra_split != tun_.lra_split || t.hash_lds != tun_.hash_lds) tables_dirty_ = true;
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
     for (DevState& g : groups_) g.remap = ds_.remap;
@@ -1649,12 +1679,13 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         };
         bms = trial(best);
         // one knob at a time, every other value of it against the best so far
-        for (int knob = 0; knob < 7; knob++) {
+        for (int knob = 0; knob < 8; knob++) {
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
             if (knob == 4 && !need_collide_) continue;
             if (knob == 5 && table_seg_.size() < 2) continue;
             if (knob == 6 && !any_lra_split_) continue;
+            if (knob == 7 && !any_hash_lds_) continue;
             const int nval = knob == 5 ? VTX_MAX_STREAMS : knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
@@ -1666,11 +1697,13 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
                 if (knob == 4) t.gather_block = 64 << v;
                 if (knob == 5) t.streams = v + 1;
                 if (knob == 6) t.lra_split = v != 0;
+                if (knob == 7) t.hash_lds = v != 0;
                 if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
                                 : knob == 2 ? t.lra_batch == base.lra_batch
                                 : knob == 3 ? t.remap == base.remap
                                 : knob == 4 ? t.gather_block == base.gather_block
-                                : knob == 5 ? t.streams == base.streams : t.lra_split == base.lra_split)
+                                : knob == 5 ? t.streams == base.streams
+                                : knob == 6 ? t.lra_split == base.lra_split : t.hash_lds == base.hash_lds)
                     continue;
                 const double ms = trial(t);
                 if (ms < 0.99 * bms) {

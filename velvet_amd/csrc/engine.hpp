@@ -126,6 +126,7 @@ struct Tuning {
                               // run on k streams, overlapping latency- and VALU-bound stages
     bool lra_split = true;    // grid segments with >= VTX_LRA_SPLIT_MIN dense slots: four threads
                               // per particle in the Jacobi pass
+    bool hash_lds = true;     // small self-colliding segments build their hash in one workgroup
 };
 
 struct StageTime {
@@ -288,6 +289,7 @@ private:
     Tuning tun_;
     bool any_lra_dense_ = false;
     bool any_lra_split_ = false;   // a stencil segment has >= VTX_LRA_SPLIT_MIN dense slots
+    bool any_hash_lds_ = false;    // a self-colliding segment is small enough for k_hash_small
     bool use_stencil_ = true;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;

@@ -44,6 +44,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
     const uint32_t g = t.begin + threadIdx.x;
+    if (sg.flags & VTX_SEG_HASH_LDS) return;
     d.vals[0][g] = g - sg.pbase;
     if (!(sg.flags & VTX_SEG_SELFCOL)) {
         d.keys[0][g] = 0;      // segment without self-collision: sorted as one cell, unused
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_clear_cells(DevState d) {
     const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
-    if (!(sg.flags & VTX_SEG_SELFCOL)) return;
+    if (!(sg.flags & VTX_SEG_SELFCOL) || (sg.flags & VTX_SEG_HASH_LDS)) return;
     uint2* __restrict__ cells = d.cells + sg.tbase;
     for (uint32_t i = t.begin - sg.pbase + threadIdx.x; i < sg.M; i += sg.n) cells[i] = make_uint2(VTX_INVALID, VTX_INVALID);
 }
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
     const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
     const SegDev sg = d.segs[t.seg];
-    if (!(sg.flags & VTX_SEG_SELFCOL)) return;
+    if (!(sg.flags & VTX_SEG_SELFCOL) || (sg.flags & VTX_SEG_HASH_LDS)) return;
     const uint32_t g = t.begin + threadIdx.x;
     const uint32_t local = g - sg.pbase;
     const uint32_t* __restrict__ sk = d.keys[src];
@@ -237,6 +238,149 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
     const float4 q = d.P[in][sg.pbase + id];
     d.sPX[2 * (size_t)g] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
     d.sPX[2 * (size_t)g + 1] = d.X0[sg.pbase + id];
+}
+
+// The whole spatial-hash build of one small segment in one workgroup of VTX_HASH_LDS_BT threads:
+// keys of the predicted positions (k_hash_keys), a stable LSD radix sort in LDS on key << 16 |
+// local index (8 bits per pass; each wave counts and then scatters its own contiguous chunk, 64
+// keys at a time ranked by wave ballots, so only three workgroup barriers per pass), the cell
+// table (k_clear_cells + k_cell_bounds) and the sorted records.
+// Same keys, same stable order, so the same table and records as the device-wide path, in one
+// launch for every small segment instead of a dozen over all of them.
+// NMAX: particles per segment at most (16384: 144 KB of LDS, one workgroup per CU). Global
+// loads are issued eight elements at a time so that their latencies overlap.
+#define VTX_HASH_LDS_BT 1024
+template <uint32_t NMAX>
+__global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, const uint32_t* __restrict__ list, int in,
+                                                                int dst) {
+    constexpr uint32_t NW = VTX_HASH_LDS_BT / 64;
+    __shared__ uint32_t buf[2][NMAX];                // key << 16 | local index
+    __shared__ uint32_t wcnt[NW][256];               // per wave: digit counts, then running offsets
+    __shared__ uint32_t wsum[4];
+    const SegDev sg = d.segs[list[blockIdx.x]];
+    const uint32_t n = sg.n, M = sg.M, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint2* __restrict__ cells = d.cells + sg.tbase;
+    for (uint32_t i = tid; i < M; i += VTX_HASH_LDS_BT) cells[i] = make_uint2(VTX_INVALID, VTX_INVALID);
+    const uint64_t mc = seg_mod(sg);
+    const float4* __restrict__ P = d.P[in] + sg.pbase;
+    constexpr uint32_t U = 8, STEP = U * VTX_HASH_LDS_BT;
+    for (uint32_t i0 = tid; i0 < n; i0 += STEP) {
+        float4 p[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
+            p[u] = i < n ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
+            if (i < n)
+                buf[0][i] = hash_coords(int_coord(p[u].x, sg.cell), int_coord(p[u].y, sg.cell),
+                                        int_coord(p[u].z, sg.cell), M, mc) << 16 | i;
+        }
+    }
+    __syncthreads();
+    const uint32_t bits = M > 1 ? 32u - (uint32_t)__clz(M - 1u) : 0u;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // wave w owns the contiguous chunk [c0, c1) of the array: the chunks in wave order and the
+    // keys in each chunk in order make the sort stable
+    const uint32_t chunk = (n + NW - 1) / NW;
+    const uint32_t c0 = min(wave * chunk, n), c1 = min(c0 + chunk, n);
+    uint32_t* __restrict__ hw = wcnt[wave];
+    uint32_t cur = 0;
+    for (uint32_t shift = 16; shift < 16 + bits; shift += 8) {
+        // 1. per-wave digit counts, 64 keys at a time (the same ballot match as below)
+        for (uint32_t k = lane; k < 256; k += 64) hw[k] = 0;
+        for (uint32_t r0 = c0; r0 < c1; r0 += 64) {
+            const uint32_t i = r0 + lane;
+            const bool valid = i < c1;
+            const uint32_t dg = valid ? (buf[cur][i] >> shift) & 255u : 0u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool bit = (dg >> b) & 1u;
+                const uint64_t bal = __ballot(valid && bit);
+                peers &= bit ? bal : ~bal;
+            }
+            if (valid && (peers & lt) == 0) hw[dg] += (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        // 2. each digit's start: digits in order, then waves in order (first 256 threads)
+        uint32_t tot = 0, incl = 0;
+        if (tid < 256) {
+            for (uint32_t w = 0; w < NW; w++) tot += wcnt[w][tid];
+            incl = wave_incl_scan(tot);
+            if (lane == 63) wsum[wave] = incl;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t run = incl - tot;
+            for (uint32_t w = 0; w < wave; w++) run += wsum[w];
+            for (uint32_t w = 0; w < NW; w++) {
+                const uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        // 3. each wave scatters its chunk in order
+        for (uint32_t r0 = c0; r0 < c1; r0 += 64) {
+            const uint32_t i = r0 + lane;
+            const bool valid = i < c1;
+            const uint32_t e = valid ? buf[cur][i] : 0u;
+            const uint32_t dg = (e >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool bit = (dg >> b) & 1u;
+                const uint64_t bal = __ballot(valid && bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            const uint32_t at = hw[dg];
+            if (valid) buf[cur ^ 1][at + rank] = e;
+            __builtin_amdgcn_wave_barrier();
+            if (valid && rank == 0) hw[dg] = at + (uint32_t)__popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // sorted keys and values, cell start / end, sorted records (as k_cell_bounds)
+    uint32_t* __restrict__ ko = d.keys[dst] + sg.pbase;
+    uint32_t* __restrict__ vo = d.vals[dst] + sg.pbase;
+    for (uint32_t g0 = tid; g0 < n; g0 += STEP) {
+        float4 q[U], x0[U];
+        uint32_t ids[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t g = g0 + u * VTX_HASH_LDS_BT;
+            ids[u] = g < n ? buf[cur][g] & 0xFFFFu : 0u;
+            q[u] = g < n ? P[ids[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            x0[u] = g < n ? d.X0[sg.pbase + ids[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t g = g0 + u * VTX_HASH_LDS_BT;
+            if (g >= n) continue;
+            const uint32_t key = buf[cur][g] >> 16, id = ids[u];
+            ko[g] = key;
+            vo[g] = id;
+            if (g == 0) {
+                cells[key].x = 0;
+            } else {
+                const uint32_t prev = buf[cur][g - 1] >> 16;
+                if (key != prev) {
+                    cells[key].x = g;
+                    cells[prev].y = g;
+                }
+            }
+            if (g == n - 1) cells[key].y = n;
+            const size_t o = 2 * (size_t)(sg.pbase + g);
+            d.sPX[o] = make_float4(q[u].x, q[u].y, q[u].z, __uint_as_float(id));
+            d.sPX[o + 1] = x0[u];
+        }
+    }
 }
 
 // CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130). Cells are visited x-, y-, z-nested;
@@ -395,6 +539,15 @@ hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s) {
     uint32_t blocks = (n + 1023) / 1024;
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(1024), 0, s, a, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_small(const DevState& d, int in, int dst, hipStream_t s) {
+    // one launch, largest segments first: workgroups of small segments fill the CUs that the
+    // large ones leave
+    if (d.nhsegs)
+        hipLaunchKernelGGL((k_hash_small<VTX_HASH_LDS_N>), dim3(d.nhsegs), dim3(VTX_HASH_LDS_BT), 0, s, d, d.hsegs, in,
+                           dst);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ hipError_t launch_sort_pass(const DevState& d, int src, int shift, hipStream_t s
 hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_clear_cells(const DevState& d, hipStream_t s);
+hipError_t launch_hash_small(const DevState& d, int in, int dst, hipStream_t s);
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s);
 // Thread -> particle order of the neighbour search (same tables for every order).
 enum { NBR_PARTICLE = 0, NBR_SORTED = 1, NBR_PATCH = 2 };

@@ -35,6 +35,11 @@
 // Attachments are "every particle to every slot, in slot order" (Velvet/VtClothObjectGPU.hpp:134-148):
 // only the distances are stored, slot-major: lra[lbase + slot * n + particle].
 #define VTX_SEG_LRA_DENSE 512u
+// The segment's spatial hash is built by one workgroup in LDS (k_hash_small): self-collision,
+// at most VTX_HASH_LDS_N particles and a table of at most 2^16 cells. The device-wide hash
+// kernels skip it.
+#define VTX_SEG_HASH_LDS 1024u
+#define VTX_HASH_LDS_N 16384
 // A particle's attachment terms are summed in four parts by slot index -- slots
 // [q*Qs, (q+1)*Qs) with Qs = ceil(scount / 4) -- each part from zero in slot order, and the parts
 // are added to the particle's delta in order q = 0..3. One lane can run all four parts, or four
@@ -137,11 +142,12 @@ struct DevState {
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
     const GridTileDev* gtiles;
     const GridTileDev* ltiles;   // grid tiles whose dense attachments run on four lanes per particle
+    const uint32_t* hsegs;       // segments whose hash is built in LDS (VTX_SEG_HASH_LDS)
     const float4* rest4;      // grid: stretch rest lengths owned by a vertex (R, D, X-diag, anti-diag)
     const uint32_t* gflags;   // grid: bending slots present at a cell (VTX_BX | VTX_BR | VTX_BD)
     const float4* brest4;     // grid: bending rest angles of a cell (X, R, D)
     uint32_t ntiles, nstiles, nhist, ncells;
-    uint32_t hbase0, nltiles, pad2, pad3;    // first histogram entry of this table set
+    uint32_t hbase0, nltiles, nhsegs, pad3;  // first histogram entry of this table set
     uint32_t nparticles, nitiles, ngtiles, remap;   // remap: workgroups -> tiles through xcd_tile
 };
 

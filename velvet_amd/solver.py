@@ -191,7 +191,8 @@ class ClothSolver:
         check(lib().vtx_set_use_graph(self.h, int(bool(on))))
 
     # Launch variants (all give bit-identical results) and the tuner that picks among them.
-    _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap", "gather_block", "streams", "lra_split")
+    _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap", "gather_block", "streams", "lra_split",
+                    "hash_lds")
 
     def tuning(self) -> dict:
         t = VtxTuning()
