@@ -1002,94 +1002,100 @@ void Engine::upload(Segment& sg, hipStream_t s) {
     take(alra_, need.n_lra, &r.lra);
     VTX_REQUIRE(r.lra + r.n_lra < (1ull << 32), VTX_ECAPACITY, "dense attachment pool exceeds 2^32 entries");
     sg.rg = r;
+    // a failed copy gives the ranges back, so that a later attempt starts clean
+    try {
+        const Prepared& pr = sg.prep;
+        const vtx_params& p = sg.prm;
+        const uint32_t n = sg.n();
+        SegDev d{};
+        d.pbase = (uint32_t)r.p;
+        d.n = n;
+        const bool selfcol = sg.selfcol();
+        d.M = selfcol ? sg.table_size() : 0;
+        VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
+        d.tbase = (uint32_t)r.cell;
+        d.nbase_lo = (uint32_t)(r.nbr & 0xFFFFFFFFu);
+        d.nbase_hi = (uint32_t)((uint64_t)r.nbr >> 32);
+        d.sbase = (uint32_t)r.slot;
+        d.scount = (uint32_t)pr.slots.size();
+        d.cbase = (uint32_t)r.col;
+        d.ccount = (uint32_t)pr.cols.size();
+        d.bbase = (uint32_t)r.brec;
+        d.trbase = (uint32_t)r.trec;
+        d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u) |
+                  (pr.lra_dense ? VTX_SEG_LRA_DENSE : 0u);
+        d.lbase = (uint32_t)r.lra;
+        d.max_nbr = (uint32_t)p.max_num_neighbors;
+        // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
+        d.diameter = p.particle_diameter;
+        d.cell = p.particle_diameter * p.hash_cell_size_scalar;
+        d.cell2 = d.cell * d.cell;
+        d.diam2 = p.particle_diameter * p.particle_diameter;
+        d.max_speed = p.max_speed;
+        d.damping = p.damping;
+        d.omega = p.relaxation_factor;
+        d.lrs = p.long_range_stretchiness;
+        d.bend_compliance = p.bend_compliance;
+        d.margin = p.collision_margin;
+        d.friction = p.friction;
+        d.stiffness = p.stretch_stiffness;
+        d.gx = p.gravity[0];
+        d.gy = p.gravity[1];
+        d.gz = p.gravity[2];
+        if (selfcol) {
+            const uint64_t c = UINT64_MAX / d.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
+            d.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
+            d.mod_hi = (uint32_t)(c >> 32);
+        }
+        d.rows = pr.grows;
+        d.cols = pr.gcols;
+        sg.dev = d;
 
-    const Prepared& pr = sg.prep;
-    const vtx_params& p = sg.prm;
-    const uint32_t n = sg.n();
-    SegDev d{};
-    d.pbase = (uint32_t)r.p;
-    d.n = n;
-    const bool selfcol = sg.selfcol();
-    d.M = selfcol ? sg.table_size() : 0;
-    VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
-    d.tbase = (uint32_t)r.cell;
-    d.nbase_lo = (uint32_t)(r.nbr & 0xFFFFFFFFu);
-    d.nbase_hi = (uint32_t)((uint64_t)r.nbr >> 32);
-    d.sbase = (uint32_t)r.slot;
-    d.scount = (uint32_t)pr.slots.size();
-    d.cbase = (uint32_t)r.col;
-    d.ccount = (uint32_t)pr.cols.size();
-    d.bbase = (uint32_t)r.brec;
-    d.trbase = (uint32_t)r.trec;
-    d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u) |
-              (pr.lra_dense ? VTX_SEG_LRA_DENSE : 0u);
-    d.lbase = (uint32_t)r.lra;
-    d.max_nbr = (uint32_t)p.max_num_neighbors;
-    // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
-    d.diameter = p.particle_diameter;
-    d.cell = p.particle_diameter * p.hash_cell_size_scalar;
-    d.cell2 = d.cell * d.cell;
-    d.diam2 = p.particle_diameter * p.particle_diameter;
-    d.max_speed = p.max_speed;
-    d.damping = p.damping;
-    d.omega = p.relaxation_factor;
-    d.lrs = p.long_range_stretchiness;
-    d.bend_compliance = p.bend_compliance;
-    d.margin = p.collision_margin;
-    d.friction = p.friction;
-    d.stiffness = p.stretch_stiffness;
-    d.gx = p.gravity[0];
-    d.gy = p.gravity[1];
-    d.gz = p.gravity[2];
-    if (selfcol) {
-        const uint64_t c = UINT64_MAX / d.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
-        d.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
-        d.mod_hi = (uint32_t)(c >> 32);
+        // Host-to-device bytes are the cost of admitting a segment: copies that repeat data already
+        // on the device are made there (predicted positions, and initial positions when they equal
+        // the positions), zero velocities are cleared, and all-zero rest angles are not read.
+        const size_t pbytes = sizeof(float4) * n;
+        h2d(pl_.X + r.p, pr.x, s);
+        if (n) {
+            VTX_HIP(hipMemcpyAsync(pl_.P0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+            VTX_HIP(hipMemcpyAsync(pl_.P1 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+            // kernels read only xyz of the initial positions, so equal xyz lets X stand in
+            bool same = pr.x0.size() == pr.x.size();
+            for (uint32_t i = 0; same && i < n; i++) same = std::memcmp(&pr.x0[i], &pr.x[i], 12) == 0;
+            if (same)
+                VTX_HIP(hipMemcpyAsync(pl_.X0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
+            else
+                h2d(pl_.X0 + r.p, pr.x0, s);
+            const bool v0 = std::all_of(pr.v.begin(), pr.v.end(), [](const float4& v) {
+                return v.x == 0.f && v.y == 0.f && v.z == 0.f && v.w == 0.f;
+            });
+            if (v0) VTX_HIP(hipMemsetAsync(pl_.V + r.p, 0, pbytes, s));
+            else h2d(pl_.V + r.p, pr.v, s);
+            VTX_HIP(hipMemsetAsync(pl_.N + r.p, 0, pbytes, s));
+        }
+        if (pr.grid) {
+            h2d(pl_.rest4 + r.p, pr.rest4, s);
+            h2d(pl_.gflags + r.p, pr.gflags, s);
+            if (!(pr.grid_flags & VTX_SEG_BEND_REST0)) h2d(pl_.brest4 + r.p, pr.brest4, s);
+        }
+        if (r.n_cell) VTX_HIP(hipMemsetAsync(pl_.cells + r.cell, 0xFF, sizeof(uint2) * r.n_cell, s));
+        if (r.n_nbr) VTX_HIP(hipMemsetAsync(pl_.nbr + r.nbr, 0xFF, sizeof(uint32_t) * r.n_nbr, s));
+        h2d(pl_.st + r.st, pr.est, s);
+        h2d(pl_.at + r.at, pr.eat, s);
+        h2d(pl_.bd + r.bd, pr.ebd, s);
+        h2d(pl_.tr + r.tr, pr.etr, s);
+        h2d(pl_.brec + r.brec, pr.brec, s);
+        h2d(pl_.brest + r.brec, pr.brest, s);
+        h2d(pl_.trec + r.trec, pr.trec, s);
+        h2d(pl_.slots + r.slot, pr.slots, s);
+        h2d(pl_.cols + r.col, pr.cols, s);
+        h2d(pl_.lra + r.lra, pr.lra, s);
+        VTX_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        sg.uploaded = true;
+        release(sg);
+        throw;
     }
-    d.rows = pr.grows;
-    d.cols = pr.gcols;
-    sg.dev = d;
-
-    // Host-to-device bytes are the cost of admitting a segment: copies that repeat data already
-    // on the device are made there (predicted positions, and initial positions when they equal
-    // the positions), zero velocities are cleared, and all-zero rest angles are not read.
-    const size_t pbytes = sizeof(float4) * n;
-    h2d(pl_.X + r.p, pr.x, s);
-    if (n) {
-        VTX_HIP(hipMemcpyAsync(pl_.P0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
-        VTX_HIP(hipMemcpyAsync(pl_.P1 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
-        // kernels read only xyz of the initial positions, so equal xyz lets X stand in
-        bool same = pr.x0.size() == pr.x.size();
-        for (uint32_t i = 0; same && i < n; i++) same = std::memcmp(&pr.x0[i], &pr.x[i], 12) == 0;
-        if (same)
-            VTX_HIP(hipMemcpyAsync(pl_.X0 + r.p, pl_.X + r.p, pbytes, hipMemcpyDeviceToDevice, s));
-        else
-            h2d(pl_.X0 + r.p, pr.x0, s);
-        const bool v0 = std::all_of(pr.v.begin(), pr.v.end(), [](const float4& v) {
-            return v.x == 0.f && v.y == 0.f && v.z == 0.f && v.w == 0.f;
-        });
-        if (v0) VTX_HIP(hipMemsetAsync(pl_.V + r.p, 0, pbytes, s));
-        else h2d(pl_.V + r.p, pr.v, s);
-        VTX_HIP(hipMemsetAsync(pl_.N + r.p, 0, pbytes, s));
-    }
-    if (pr.grid) {
-        h2d(pl_.rest4 + r.p, pr.rest4, s);
-        h2d(pl_.gflags + r.p, pr.gflags, s);
-        if (!(pr.grid_flags & VTX_SEG_BEND_REST0)) h2d(pl_.brest4 + r.p, pr.brest4, s);
-    }
-    if (r.n_cell) VTX_HIP(hipMemsetAsync(pl_.cells + r.cell, 0xFF, sizeof(uint2) * r.n_cell, s));
-    if (r.n_nbr) VTX_HIP(hipMemsetAsync(pl_.nbr + r.nbr, 0xFF, sizeof(uint32_t) * r.n_nbr, s));
-    h2d(pl_.st + r.st, pr.est, s);
-    h2d(pl_.at + r.at, pr.eat, s);
-    h2d(pl_.bd + r.bd, pr.ebd, s);
-    h2d(pl_.tr + r.tr, pr.etr, s);
-    h2d(pl_.brec + r.brec, pr.brec, s);
-    h2d(pl_.brest + r.brec, pr.brest, s);
-    h2d(pl_.trec + r.trec, pr.trec, s);
-    h2d(pl_.slots + r.slot, pr.slots, s);
-    h2d(pl_.cols + r.col, pr.cols, s);
-    h2d(pl_.lra + r.lra, pr.lra, s);
-    VTX_HIP(hipStreamSynchronize(s));
     sg.uploaded = true;
 }
 

@@ -84,14 +84,20 @@ uint32_t Server::admit(bool stage, uint32_t before) {
     while (!queue_.empty()) {
         Job& j = get(queue_.front());
         if (!fits(j, before + n)) break;
-        queue_.pop_front();
         if (stage) {
-            e_.stage(j.seg);
+            // a frame is in flight: an upload that fails (e.g. device memory) leaves the job
+            // queued and ends this step's admission, so the step's own accounting still runs
+            try {
+                e_.stage(j.seg);
+            } catch (const Error&) {
+                break;
+            }
             staged_.push_back(j.id);
         } else {
             e_.set_active(j.seg, true);
             live_.push_back(j.id);
         }
+        queue_.pop_front();
         j.status = VTX_JOB_RUNNING;
         resident_ += j.bytes;
         live_particles_ += j.n;
