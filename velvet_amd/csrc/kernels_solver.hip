@@ -348,9 +348,9 @@ __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float
     const V3 p0 = xyz(q0), p1 = xyz(q1), p2 = xyz(q2), p3 = xyz(q3);
     const V3 e = p3 - p2;
     const float ee = dot(e, e);
-    const float elen = fsqrt(ee);
-    if (elen < VTX_EPSILON) return false;
-    const float inv_elen = frcp(elen);
+    if (ee < VTX_EPSILON * VTX_EPSILON) return false;   // |e| < eps
+    const float inv_elen = frsq(ee);                      // |e| and 1/|e| from one rsq
+    const float elen = ee * inv_elen;
     const V3 a = p2 - p0, b = p3 - p1;
     const V3 c1 = cross(a, e);
     const V3 c2 = cross(e, b);
