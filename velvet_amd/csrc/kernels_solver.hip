@@ -340,11 +340,20 @@ __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg
 //   c1 = (p2-p0) x (p3-p0) = a x e,   c2 = (p3-p1) x (p2-p1) = e x b,
 //   (p0-p3).e = -(a.e) - |e|^2,        (p1-p3).e = -(b.e),
 //   (c1 x c2).e = |e|^2 (a.c2)         (since e.c2 = 0), so the sign rule needs one dot product.
-// The reference's n1 = c1 / |c1|^2 and normalize(n1) = c1 / |c1| come from one reciprocal square
-// root each. Returns false on the reference's early-outs; else lam carries the sign rule and the
-// correction of vertex k is -w_k * lam * d_k.
+// With r1 = 1/|c1| and r2 = 1/|c2| (one reciprocal square root each), the reference's
+// n1 = c1 r1^2, n2 = c2 r2^2 and normalize(n1).normalize(n2) = (c1.c2) r1 r2, so every gradient is
+// a combination of c1 and c2: d0 = k0 c1, d1 = k1 c2 (k0 = |e| r1^2, k1 = |e| r2^2) and
+// d2 = s1 c1 + s2 c2, d3 = -(d0 + d1 + d2) = t1 c1 + t2 c2 (the four gradients of a dihedral
+// angle sum to zero; the reference's d3 = (p2-p0).e/|e| n1 + (p2-p1).e/|e| n2 is the same).
+// Returns false on the reference's early-outs; else lam carries the sign rule and the correction
+// of vertex k is -w_k * lam * d_k.
+struct BendGrad {
+    V3 c1, c2, d2, d3;
+    float k0, k1;
+};
+
 __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float4 q3, float rest, float xpbd,
-                                          float& lam, V3& d0, V3& d1, V3& d2, V3& d3) {
+                                          float& lam, BendGrad& g) {
     const V3 p0 = xyz(q0), p1 = xyz(q1), p2 = xyz(q2), p3 = xyz(q3);
     const V3 e = p3 - p2;
     const float ee = dot(e, e);
@@ -352,24 +361,22 @@ __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float
     const float inv_elen = frsq(ee);                      // |e| and 1/|e| from one rsq
     const float elen = ee * inv_elen;
     const V3 a = p2 - p0, b = p3 - p1;
-    const V3 c1 = cross(a, e);
-    const V3 c2 = cross(e, b);
-    const float r1 = frsq(dot(c1, c1)), r2 = frsq(dot(c2, c2));
-    const V3 u1 = c1 * r1, u2 = c2 * r2;
-    const V3 n1 = u1 * r1, n2 = u2 * r2;
-    d0 = elen * n1;
-    d1 = elen * n2;
-    d2 = ((-dot(a, e) - ee) * inv_elen) * n1 + (-dot(b, e) * inv_elen) * n2;
-    // the four gradients of a dihedral angle sum to zero (it is translation invariant):
-    // the reference's d3 = (p2-p0).e/|e| n1 + (p2-p1).e/|e| n2 equals -(d0 + d1 + d2)
-    d3 = -(d0 + d1 + d2);
-    const float cd = fminf(fmaxf(dot(u1, u2), -1.0f), 1.0f);
+    g.c1 = cross(a, e);
+    g.c2 = cross(e, b);
+    const float r1 = frsq(dot(g.c1, g.c1)), r2 = frsq(dot(g.c2, g.c2));
+    const float r1s = r1 * r1, r2s = r2 * r2;
+    g.k0 = elen * r1s;
+    g.k1 = elen * r2s;
+    const float s1 = ((-dot(a, e) - ee) * inv_elen) * r1s, s2 = (-dot(b, e) * inv_elen) * r2s;
+    g.d2 = s1 * g.c1 + s2 * g.c2;
+    g.d3 = (-(g.k0 + s1)) * g.c1 + (-(g.k1 + s2)) * g.c2;
+    const float cd = fminf(fmaxf(dot(g.c1, g.c2) * (r1 * r2), -1.0f), 1.0f);
     const float phi = facos(cd);
-    // |d0|^2 = |e|^2 / |c1|^2 and |d1|^2 = |e|^2 / |c2|^2
-    float l = ee * (q0.w * r1 * r1 + q1.w * r2 * r2) + q2.w * dot(d2, d2) + q3.w * dot(d3, d3);
+    // |d0|^2 = |e|^2 r1^2 and |d1|^2 = |e|^2 r2^2
+    float l = ee * (q0.w * r1s + q1.w * r2s) + q2.w * dot(g.d2, g.d2) + q3.w * dot(g.d3, g.d3);
     if (l < VTX_EPSILON) return false;
     l = (phi - rest) * frcp(l + xpbd);
-    if (dot(a, c2) > 0.0f) l = -l;      // (n1 x n2).e > 0
+    if (dot(a, g.c2) > 0.0f) l = -l;      // (n1 x n2).e > 0
     lam = l;
     return true;
 }
@@ -435,11 +442,10 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             const uint4 ix = d.brec[bid];
             const uint32_t pb = sg.pbase;
             float lam;
-            V3 d0, d1, d2, d3;
-            if (!bend_eval(P[pb + ix.x], P[pb + ix.y], P[pb + ix.z], P[pb + ix.w], d.brest[bid], xpbd_bend, lam, d0,
-                           d1, d2, d3))
+            BendGrad bg;
+            if (!bend_eval(P[pb + ix.x], P[pb + ix.y], P[pb + ix.z], P[pb + ix.w], d.brest[bid], xpbd_bend, lam, bg))
                 continue;
-            const V3 dk = role == 0 ? d0 : role == 1 ? d1 : role == 2 ? d2 : d3;
+            const V3 dk = role == 0 ? bg.k0 * bg.c1 : role == 1 ? bg.k1 * bg.c2 : role == 2 ? bg.d2 : bg.d3;
             delta += (-w * lam) * dk;
             cnt++;
         }
@@ -505,11 +511,11 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
             rest = TYPE == 0 ? r4.x : TYPE == 1 ? r4.y : r4.z;
         }
         float lam;
-        V3 d0, d1, d2, d3;
-        on = bend_eval(q0, q1, q2, q3, rest, xpbd, lam, d0, d1, d2, d3);
+        BendGrad bg;
+        on = bend_eval(q0, q1, q2, q3, rest, xpbd, lam, bg);
         if (on) {
-            const V3 c0 = (-q0.w * lam) * d0, c1 = (-q1.w * lam) * d1, c2 = (-q2.w * lam) * d2,
-                     c3 = (-q3.w * lam) * d3;
+            const V3 c0 = (-q0.w * lam * bg.k0) * bg.c1, c1 = (-q1.w * lam * bg.k1) * bg.c2,
+                     c2 = (-q2.w * lam) * bg.d2, c3 = (-q3.w * lam) * bg.d3;
             corr[0][j] = c0.x; corr[1][j] = c0.y; corr[2][j] = c0.z;
             corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
             corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
