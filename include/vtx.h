@@ -76,9 +76,14 @@ typedef struct vtx_collider {
 
 typedef struct vtx_solver vtx_solver;
 
+/* Layout version of the structs above and below; vtx_abi_version() returns the library's, so a
+ * binding can check at load time that it was written against the same layouts. */
+#define VTX_ABI_VERSION 2
+
 void vtx_params_default(vtx_params* p);
 const char* vtx_last_error(void);
 const char* vtx_version(void);
+int vtx_abi_version(void);
 
 /* Create / destroy. Replaces VtClothSolverGPU construction and Start()
  * (Velvet/VtClothSolverGPU.hpp:27-33) and OnDestroy (hpp:50-54). */

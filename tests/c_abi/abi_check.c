@@ -48,5 +48,6 @@ int main(void) {
     printf("small rc %d %s\n", rc, rc ? "error" : "ok");
     printf("message %s\n", vtx_last_error());
     printf("version %s\n", vtx_version());
+    printf("abi %d %d\n", vtx_abi_version(), VTX_ABI_VERSION);
     return 0;
 }

@@ -68,3 +68,4 @@ def test_mesh_generator_through_c_matches_the_binding(c_output):
     assert np.any(br > 1e-3) and np.any(br == 0.0)
     assert _field(c_output, "small rc") == ["-5 error"]            # VTX_ECAPACITY
     assert "gfx950" in _field(c_output, "version")[0]
+    assert _field(c_output, "abi") == [f"{_lib.ABI_VERSION} {_lib.ABI_VERSION}"]

@@ -25,6 +25,13 @@ def test_defaults_are_the_reference_defaults():
     assert b"gfx950" in _lib.lib().vtx_version()
 
 
+def test_abi_version_matches_header():
+    import re
+    txt = open(_lib.HEADER).read()
+    assert int(re.search(r"#define VTX_ABI_VERSION (\d+)", txt).group(1)) == _lib.ABI_VERSION
+    assert _lib.lib().vtx_abi_version() == _lib.ABI_VERSION
+
+
 def test_struct_sizes_match_header():
     assert C.sizeof(_lib.VtxParams) == 4 * 20
     assert C.sizeof(_lib.VtxCollider) == 4 * (1 + 3 + 3 + 16 + 16 + 1)

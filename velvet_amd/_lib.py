@@ -17,6 +17,9 @@ i32, u32, f32 = C.c_int32, C.c_uint32, C.c_float
 P = C.POINTER
 
 
+ABI_VERSION = 2   # VTX_ABI_VERSION of include/vtx.h for the structures below
+
+
 class VtxParams(C.Structure):
     _fields_ = [("num_substeps", i32), ("num_iterations", i32), ("max_num_neighbors", i32),
                 ("max_speed", f32), ("gravity", f32 * 3), ("bend_compliance", f32),
@@ -55,6 +58,7 @@ _SIGS = {
     "vtx_params_default": (None, [P(VtxParams)]),
     "vtx_last_error": (C.c_char_p, []),
     "vtx_version": (C.c_char_p, []),
+    "vtx_abi_version": (i32, []),
     "vtx_solver_create": (i32, [P(VtxParams), P(C.c_void_p)]),
     "vtx_solver_destroy": (i32, [C.c_void_p]),
     "vtx_set_params": (i32, [C.c_void_p, i32, P(VtxParams)]),
@@ -141,6 +145,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.vtx_abi_version() != ABI_VERSION:   # the structures below follow this layout version
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.vtx_abi_version()}, this binding expects "
+                               f"{ABI_VERSION}: rebuild with `python velvet_amd/build.py`")
         _LIB = L
     return _LIB
 

@@ -78,7 +78,8 @@ void vtx_params_default(vtx_params* p) {
 }
 
 const char* vtx_last_error(void) { return g_error.c_str(); }
-const char* vtx_version(void) { return "vtx 0.1 gfx950"; }
+const char* vtx_version(void) { return "vtx 0.2 gfx950"; }
+int vtx_abi_version(void) { return VTX_ABI_VERSION; }
 
 int vtx_solver_create(const vtx_params* params, vtx_solver** out) {
     NEED(out);
