@@ -135,7 +135,10 @@ int vtx_add_bend_n(vtx_solver* s, int32_t seg, const uint32_t* idx, const float*
 int vtx_set_colliders(vtx_solver* s, int32_t seg, const vtx_collider* cols, uint32_t n);
 
 /* Replaces Simulate() (Velvet/VtClothSolverGPU.hpp:56-111): one frame of every segment.
- * Asynchronous on the solver's stream; vtx_synchronize waits for it. */
+ * Asynchronous on the solver's stream; vtx_synchronize waits for it. On a stream bound with
+ * vtx_set_stream, work queued on that stream after the call sees the whole frame. On the
+ * solver's own stream the extra stream groups (vtx_tuning.streams) may run into the next frame;
+ * every other call that touches device state waits for them first. */
 int vtx_simulate(vtx_solver* s, float frame_dt);
 int vtx_synchronize(vtx_solver* s);
 

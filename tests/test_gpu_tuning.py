@@ -7,7 +7,9 @@
   segment);
 * vtx_autotune leaves the simulation state where it was (trials are undone) and a later run
   under the tuned variant equals an untuned run bit for bit;
-* a second solver of the same shape class takes the cached tuning without timing.
+* a second solver of the same shape class takes the cached tuning without timing;
+* host operations queued between frames (slot moves, colliders, particle edits, readback,
+  snapshot / restore) give the same results with 1 to 4 stream groups.
 """
 import sys
 
@@ -93,6 +95,48 @@ def test_lra_split_in_a_mixed_batch():
         got = _run(scenes, 3, **v)
         for k in range(len(scenes)):
             assert np.array_equal(got[k][0], alone[k][0]) and np.array_equal(got[k][1], alone[k][1]), (k, v)
+
+
+def _run_with_host_ops(scenes, **tuning):
+    """Frames without a synchronize between them, with every kind of host operation that
+    touches device state queued between two frames."""
+    with ClothSolver(use_graph=False) as s:
+        segs = [s.load_scene(sc) for sc in scenes]
+        s.set_tuning(**tuning)
+        snap = None
+        for f in range(6):
+            s.Simulate()
+            if f == 0:
+                for k, sc in enumerate(scenes):
+                    if sc.cloths[0].slots.shape[0]:
+                        s.SetAttachSlotPosition(0, sc.cloths[0].slots[0] + np.float32([0.05, 0.0, 0.0]), seg=segs[k])
+            elif f == 1:
+                s.UpdateColliders(list(scenes[-1].colliders) + [S.sphere((0.0, 0.6, 0.0), 0.3)], seg=segs[-1])
+            elif f == 2:
+                s.set_particle(3, pos=[0.0, 1.2, 0.0], vel=[0.0, 0.0, 0.0], seg=segs[-1])
+            elif f == 3:
+                s.readback_request(segs[2])
+                snap = s.snapshot(segs[4])
+            elif f == 4:
+                s.restore(snap, seg=segs[4])
+        s.synchronize()
+        rb = s.readback_latest(segs[2])
+        return [(s.positions(k), s.velocities(k)) for k in segs], snap, rb
+
+
+def test_host_operations_between_frames_with_stream_groups():
+    """On the solver's own stream the stream groups are joined lazily; every host operation
+    between frames must still see (and be seen by) all of them, as with a single stream."""
+    scenes = _batch()
+    ref, snap_ref, rb_ref = _run_with_host_ops(scenes, streams=1)
+    for k in (2, 3, 4):
+        say(f"host operations between frames, streams={k}")
+        got, snap, rb = _run_with_host_ops(scenes, streams=k)
+        assert snap == snap_ref, k
+        assert rb is not None and rb_ref is not None and rb[2] == rb_ref[2]
+        assert np.array_equal(rb[0], rb_ref[0]) and np.array_equal(rb[1], rb_ref[1]), k
+        for j, ((x, v), (xr, vr)) in enumerate(zip(got, ref)):
+            assert np.array_equal(x, xr) and np.array_equal(v, vr), (k, j)
 
 
 def test_autotune_keeps_state_and_caches():

@@ -262,6 +262,7 @@ Engine::Engine(const vtx_params& p) {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipStream_t x : xstreams_) (void)hipStreamSynchronize(x);
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     for (auto& st : stages_) {
         (void)hipEventDestroy(st.a);
@@ -272,10 +273,7 @@ Engine::~Engine() {
     if (hist_) (void)hipFree(hist_);
     if (part_) (void)hipFree(part_);
     if (flags_) (void)hipFree(flags_);
-    for (hipStream_t x : xstreams_) {
-        (void)hipStreamSynchronize(x);
-        (void)hipStreamDestroy(x);
-    }
+    for (hipStream_t x : xstreams_) (void)hipStreamDestroy(x);
     if (cstream_) {
         (void)hipStreamSynchronize(cstream_);
         (void)hipStreamDestroy(cstream_);
@@ -320,6 +318,7 @@ int32_t Engine::add_segment(const vtx_params& p, bool active) {
 }
 
 void Engine::remove_segment(int32_t s) {
+    join();
     Segment& sg = seg(s);
     if (sg.uploaded) {
         synchronize();
@@ -330,6 +329,7 @@ void Engine::remove_segment(int32_t s) {
 }
 
 void Engine::set_active(int32_t s, bool active) {
+    join();
     Segment& sg = seg(s);
     if (sg.active == active) return;
     sg.active = active;
@@ -339,6 +339,7 @@ void Engine::set_active(int32_t s, bool active) {
 // Deactivate a segment and give its device memory back; its state goes to the host and its
 // prepared layout is kept, so re-activation only uploads.
 void Engine::park(int32_t s) {
+    join();
     Segment& sg = seg(s);
     if (sg.uploaded) {
         pull_state(sg);
@@ -368,6 +369,7 @@ size_t Engine::snapshot_bytes(int32_t s) {
 }
 
 void Engine::snapshot(int32_t s, void* buf, size_t bytes) {
+    join();
     Segment& sg = seg(s);
     const size_t need = snapshot_bytes(s);
     VTX_REQUIRE(buf && bytes >= need, VTX_EINVAL, "snapshot buffer too small");
@@ -394,6 +396,7 @@ void Engine::snapshot(int32_t s, void* buf, size_t bytes) {
 }
 
 void Engine::restore(int32_t s, const void* buf, size_t bytes) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(buf && bytes >= sizeof(SnapHeader), VTX_EINVAL, "snapshot buffer too small");
     SnapHeader h;
@@ -439,6 +442,7 @@ Readback::~Readback() {
 }
 
 void Engine::readback_request(int32_t s) {
+    join();
     Segment& sg = seg(s);
     ensure_committed();
     VTX_REQUIRE(sg.active && sg.uploaded, VTX_ESTATE, "segment is not on the device");
@@ -492,7 +496,7 @@ void Engine::invalidate_graph() {
 // Device -> host copy of one segment's particle state.
 void Engine::pull_state(Segment& sg) {
     if (!sg.uploaded) return;
-    VTX_HIP(hipStreamSynchronize(stream_));
+    sync_stream();
     const uint32_t n = sg.n();
     if (!n) return;
     std::vector<float4> tmp(n);
@@ -524,6 +528,7 @@ void Engine::touch(int32_t s) {
 }
 
 void Engine::set_params(int32_t s, const vtx_params& p) {
+    join();
     validate_params(p);
     touch(s);
     seg(s).prm = p;
@@ -532,6 +537,7 @@ void Engine::set_params(int32_t s, const vtx_params& p) {
 // AddCloth (Velvet/VtClothSolverGPU.hpp:114-156).
 int32_t Engine::add_cloth(int32_t s, const float* xyz, uint32_t n, const uint32_t* tris, uint32_t ntri,
                           const float* model, float diameter) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(n == 0 || xyz, VTX_EINVAL, "positions are NULL");
     VTX_REQUIRE(ntri == 0 || tris, VTX_EINVAL, "triangles are NULL");
@@ -564,6 +570,7 @@ int32_t Engine::add_cloth(int32_t s, const float* xyz, uint32_t n, const uint32_
 }
 
 void Engine::add_stretch(int32_t s, const int32_t* ij, const float* rest, uint32_t count) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(count == 0 || (ij && rest), VTX_EINVAL, "stretch arrays are NULL");
     const int32_t n = (int32_t)sg.n();
@@ -575,6 +582,7 @@ void Engine::add_stretch(int32_t s, const int32_t* ij, const float* rest, uint32
 }
 
 int32_t Engine::add_attach_slot(int32_t s, const float* pos) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
     touch(s);
@@ -583,6 +591,7 @@ int32_t Engine::add_attach_slot(int32_t s, const float* pos) {
 }
 
 void Engine::set_attach_slot_position(int32_t s, int32_t slot, const float* pos) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(pos, VTX_EINVAL, "slot position is NULL");
     VTX_REQUIRE(slot >= 0 && (size_t)slot < sg.slots.size() / 3, VTX_EINVAL, "slot index out of range");
@@ -591,12 +600,13 @@ void Engine::set_attach_slot_position(int32_t s, int32_t slot, const float* pos)
     if (sg.prepared) sg.prep.slots[(size_t)slot] = v;
     if (sg.uploaded) {   // the graph reads the slot table: no re-record
         VTX_HIP(hipMemcpyAsync(pl_.slots + sg.rg.slot + slot, &v, sizeof(float4), hipMemcpyHostToDevice, stream_));
-        VTX_HIP(hipStreamSynchronize(stream_));
+        sync_stream();
     }
 }
 
 // AddAttach (Velvet/VtClothSolverGPU.hpp:170-176): distance 0 pins the particle.
 void Engine::add_attach(int32_t s, const int32_t* pid, const int32_t* slot, const float* dist, uint32_t count) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(count == 0 || (pid && slot && dist), VTX_EINVAL, "attach arrays are NULL");
     const int32_t n = (int32_t)sg.n();
@@ -615,6 +625,7 @@ void Engine::add_attach(int32_t s, const int32_t* pid, const int32_t* slot, cons
 }
 
 void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(count == 0 || (idx && angle), VTX_EINVAL, "bend arrays are NULL");
     VTX_REQUIRE(sg.bd.size() / 4 + count < (1u << 28), VTX_ECAPACITY, "too many bend constraints");
@@ -625,6 +636,7 @@ void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32
 }
 
 void Engine::add_mesh_constraints(int32_t s, int32_t offset, const uint32_t* tris, uint32_t ntri) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(ntri == 0 || tris, VTX_EINVAL, "triangles are NULL");
     VTX_REQUIRE(offset >= 0 && (uint32_t)offset <= sg.n(), VTX_EINVAL, "cloth offset out of range");
@@ -643,6 +655,7 @@ void Engine::add_mesh_constraints(int32_t s, int32_t offset, const uint32_t* tri
 }
 
 void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(n == 0 || cols, VTX_EINVAL, "colliders are NULL");
     for (uint32_t c = 0; c < n; c++)
@@ -653,7 +666,7 @@ void Engine::set_colliders(int32_t s, const vtx_collider* cols, uint32_t n) {
         if (sg.uploaded && n) {
             VTX_HIP(hipMemcpyAsync(pl_.cols + sg.rg.col, sg.prep.cols.data(), sizeof(ColliderDev) * n,
                                    hipMemcpyHostToDevice, stream_));
-            VTX_HIP(hipStreamSynchronize(stream_));
+            sync_stream();
         }
         return;
     }
@@ -945,18 +958,20 @@ bool Engine::pools_fit(const Ranges& need) {
 }
 
 void Engine::stage(int32_t s) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(!sg.active, VTX_ESTATE, "only an inactive segment can be staged");
     if (sg.uploaded) return;
     prepare(s);
-    if (!pools_fit(needs_of(sg))) VTX_HIP(hipStreamSynchronize(stream_));   // growth moves every pool
+    if (!pools_fit(needs_of(sg))) sync_stream();   // growth moves every pool
     if (!cstream_) VTX_HIP(hipStreamCreateWithFlags(&cstream_, hipStreamNonBlocking));
     upload(sg, cstream_);
 }
 
 void Engine::reserve(uint64_t particles) {
+    join();
     if (!particles) return;
-    VTX_HIP(hipStreamSynchronize(stream_));
+    sync_stream();
     Ranges need;
     need.n_p = particles;
     need.n_cell = 2 * particles;                              // default table size 2 n
@@ -1171,7 +1186,7 @@ static uint32_t ceil_log2(uint32_t v) {
 
 // Segment, tile and sort tables of the active segments.
 void Engine::rebuild_tables() {
-    VTX_HIP(hipStreamSynchronize(stream_));
+    sync_stream();
     invalidate_graph();
     std::vector<SegDev> hseg;
     std::vector<TileDev> htile, hitile;
@@ -1405,7 +1420,7 @@ void Engine::ensure_committed() {
         any = true;
     }
     if (any) {
-        VTX_HIP(hipStreamSynchronize(stream_));
+        sync_stream();
         grow_pools(need);
         for (auto& sg : segs_)
             if (sg && sg->active && !sg->uploaded) upload(*sg, stream_);
@@ -1416,6 +1431,7 @@ void Engine::ensure_committed() {
 
 // ---------------------------------------------------------------------------- grid path switch
 void Engine::set_stencil_path(bool on) {
+    join();
     if (on == use_stencil_) return;
     for (size_t si = 0; si < segs_.size(); si++)
         if (segs_[si]) touch((int32_t)si);
@@ -1481,14 +1497,18 @@ void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
 
 // VtClothSolverGPU::Simulate stage order (Velvet/VtClothSolverGPU.hpp:56-111) for one table set.
 // In timing mode every launch group is bracketed by an event pair labelled with its kernel.
-void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s) {
+void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s, hipEvent_t after_hash) {
     const float h = frame_dt / (float)substeps_;
     stage_begin("k_frame_begin", s);
     VTX_HIP(launch_frame_begin(d, frame_dt, h, s));
     stage_end(s);
+    if (after_hash && !any_selfcol_) VTX_HIP(hipEventRecord(after_hash, s));
     int cur = 0;
     for (int sub = 0; sub < substeps_; sub++) {
-        if (any_selfcol_ && sub % interleave_ == 0) hash_build(d, cur, s);
+        if (any_selfcol_ && sub % interleave_ == 0) {
+            hash_build(d, cur, s);
+            if (after_hash && sub == 0) VTX_HIP(hipEventRecord(after_hash, s));
+        }
         if (need_collide_) {
             stage_begin("k_collide", s);
             VTX_HIP(launch_collide(d, cur, h, tun_.gather_block, s));
@@ -1516,22 +1536,35 @@ void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s) {
     stage_end(s);
 }
 
-// One frame: with several table sets, group k > 0 runs on stream k of its own, forked from and
-// joined back into the solver's stream (also inside a graph capture), so the groups' segments
-// advance concurrently and every later operation on the solver's stream sees all of them.
-void Engine::record_frame(float frame_dt, hipStream_t s) {
+// One frame: with several table sets, group k > 0 runs on stream k of its own. Joined (graph
+// capture, timing): forked from and joined back into the solver's stream, so every later
+// operation on it sees all groups. Lazy: the extra streams start once group 0 has built its
+// hash, so that their hash and neighbour phases overlap group 0's iterations rather than its
+// own, and the solver's stream does not wait for them at the end of the frame; join() adds
+// that wait, and every operation other than the next frame that touches device state calls it.
+void Engine::record_frame(float frame_dt, hipStream_t s, bool lazy) {
     if (groups_.size() < 2) {
         record_group(groups_.empty() ? ds_ : groups_[0], frame_dt, s);
         return;
     }
-    VTX_HIP(hipEventRecord(ev_fork_, s));
-    for (size_t k = 1; k < groups_.size(); k++) VTX_HIP(hipStreamWaitEvent(xstreams_[k - 1], ev_fork_, 0));
-    record_group(groups_[0], frame_dt, s);
+    if (lazy) {
+        record_group(groups_[0], frame_dt, s, ev_fork_);
+    } else {
+        VTX_HIP(hipEventRecord(ev_fork_, s));
+        record_group(groups_[0], frame_dt, s);
+    }
     for (size_t k = 1; k < groups_.size(); k++) {
+        VTX_HIP(hipStreamWaitEvent(xstreams_[k - 1], ev_fork_, 0));
         record_group(groups_[k], frame_dt, xstreams_[k - 1]);
         VTX_HIP(hipEventRecord(ev_join_[k - 1], xstreams_[k - 1]));
     }
-    for (size_t k = 1; k < groups_.size(); k++) VTX_HIP(hipStreamWaitEvent(s, ev_join_[k - 1], 0));
+    pending_join_ = groups_.size() - 1;
+    if (!lazy) join();
+}
+
+void Engine::join() {
+    for (size_t k = 0; k < pending_join_; k++) VTX_HIP(hipStreamWaitEvent(stream_, ev_join_[k], 0));
+    pending_join_ = 0;
 }
 
 void Engine::simulate(float frame_dt) {
@@ -1541,8 +1574,9 @@ void Engine::simulate(float frame_dt) {
     frames_++;
     if (timing_) {
         stage_used_ = 0;
-        record_frame(frame_dt, stream_);
-        VTX_HIP(hipStreamSynchronize(stream_));
+        join();
+        record_frame(frame_dt, stream_, false);
+        sync_stream();
         for (size_t k = 0; k < stage_used_; k++) {
             float ms = 0.f;
             VTX_HIP(hipEventElapsedTime(&ms, stages_[k].a, stages_[k].b));
@@ -1555,9 +1589,10 @@ void Engine::simulate(float frame_dt) {
         if (!gexec_ || graph_dt_ != frame_dt) {
             invalidate_graph();
             hipGraph_t graph = nullptr;
+            join();
             VTX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             try {
-                record_frame(frame_dt, stream_);
+                record_frame(frame_dt, stream_, false);
             } catch (...) {
                 (void)hipStreamEndCapture(stream_, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1569,20 +1604,29 @@ void Engine::simulate(float frame_dt) {
             VTX_HIP(e);
             graph_dt_ = frame_dt;
         }
+        join();
         VTX_HIP(hipGraphLaunch(gexec_, stream_));
     } else {
-        record_frame(frame_dt, stream_);
+        // a caller's stream keeps its order: work queued on it after the frame sees every group
+        record_frame(frame_dt, stream_, own_stream_);
     }
 }
 
-void Engine::synchronize() { VTX_HIP(hipStreamSynchronize(stream_)); }
+void Engine::sync_stream() {
+    join();
+    VTX_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::synchronize() { sync_stream(); }
 
 void Engine::set_use_graph(bool on) {
+    join();
     tun_.graph = on;
     invalidate_graph();
 }
 
 void Engine::set_tuning(const Tuning& t) {
+    join();
     VTX_REQUIRE(t.lra_batch == 4 || t.lra_batch == 16 || t.lra_batch == 64, VTX_EINVAL,
                 "lra_batch must be 4, 16 or 64");
     VTX_REQUIRE(t.nbr_order >= 0 && t.nbr_order <= 2, VTX_EINVAL, "neighbor_order must be 0, 1 or 2");
@@ -1624,6 +1668,7 @@ std::map<std::string, TunedClass> g_tuned;   // shape class -> tuning, shared by
 }  // namespace
 
 Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* frame_ms) {
+    join();
     VTX_REQUIRE(frames >= 1 && frame_dt > 0.f, VTX_EINVAL, "autotune needs frames >= 1 and frame_dt > 0");
     ensure_committed();
     if (frame_ms) *frame_ms = 0.0;
@@ -1644,6 +1689,8 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
     const uint64_t frames_before = frames_;
     timing_ = false;
     auto cleanup = [&] {
+        for (size_t k = 0; k < pending_join_; k++) (void)hipStreamSynchronize(xstreams_[k]);
+        pending_join_ = 0;
         (void)hipStreamSynchronize(stream_);
         if (sx) (void)hipFree(sx);
         if (sv) (void)hipFree(sv);
@@ -1666,6 +1713,7 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
         VTX_HIP(hipMemcpyAsync(sv, pl_.V, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         VTX_HIP(hipMemcpyAsync(sn, pl_.N, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         auto restore = [&] {
+            join();
             VTX_HIP(hipMemcpyAsync(pl_.X, sx, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             VTX_HIP(hipMemcpyAsync(pl_.V, sv, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             VTX_HIP(hipMemcpyAsync(pl_.N, sn, n * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
@@ -1676,9 +1724,10 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
             restore();
             VTX_HIP(hipEventRecord(e0, stream_));
             for (int f = 0; f < frames; f++) simulate(frame_dt);
+            join();
             VTX_HIP(hipEventRecord(e1, stream_));
             restore();
-            VTX_HIP(hipStreamSynchronize(stream_));
+            sync_stream();
             float ms = 0.f;
             VTX_HIP(hipEventElapsedTime(&ms, e0, e1));
             return (double)ms / frames;
@@ -1733,6 +1782,7 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
 }
 
 void Engine::set_stream(hipStream_t s) {
+    join();
     synchronize();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     if (s) {
@@ -1746,6 +1796,7 @@ void Engine::set_stream(hipStream_t s) {
 }
 
 void Engine::set_timing(bool on) {
+    join();
     if (on != timing_) tables_dirty_ = true;   // timing mode runs all segments as one table set
     timing_ = on;
     timing_ms_.clear();
@@ -1768,6 +1819,7 @@ std::string Engine::timing_report() {
 
 // ---------------------------------------------------------------------------- state access
 void Engine::read_particles(int32_t s, int what, float* out) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(out, VTX_EINVAL, "output is NULL");
     ensure_committed();
@@ -1798,6 +1850,7 @@ void Engine::read_particles(int32_t s, int what, float* out) {
 }
 
 float4* Engine::device_positions(int32_t s, uint32_t* n) {
+    join();
     Segment& sg = seg(s);
     ensure_committed();
     VTX_REQUIRE(sg.uploaded, VTX_ESTATE, "segment is not resident on the device");
@@ -1806,6 +1859,7 @@ float4* Engine::device_positions(int32_t s, uint32_t* n) {
 }
 
 float4* Engine::device_normals(int32_t s, uint32_t* n) {
+    join();
     Segment& sg = seg(s);
     ensure_committed();
     VTX_REQUIRE(sg.uploaded, VTX_ESTATE, "segment is not resident on the device");
@@ -1815,6 +1869,7 @@ float4* Engine::device_normals(int32_t s, uint32_t* n) {
 
 // MouseGrabber-style edit (Velvet/MouseGrabber.hpp:51-52,60,74-75).
 void Engine::set_particle(int32_t s, int32_t i, const float* pos, const float* vel, float inv_mass) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(i >= 0 && (uint32_t)i < sg.n(), VTX_EINVAL, "particle index out of range");
     if (!sg.uploaded) {
@@ -1842,6 +1897,7 @@ void Engine::set_particle(int32_t s, int32_t i, const float* pos, const float* v
 }
 
 void Engine::write_predicted(int32_t s, const float* xyz) {
+    join();
     Segment& sg = seg(s);
     VTX_REQUIRE(xyz, VTX_EINVAL, "positions are NULL");
     ensure_committed();
@@ -1860,6 +1916,7 @@ void Engine::write_predicted(int32_t s, const float* xyz) {
 }
 
 void Engine::run_hash() {
+    join();
     ensure_committed();
     VTX_REQUIRE(any_selfcol_, VTX_ESTATE, "no active segment has self collision enabled");
     for (const DevState& g : groups_) hash_build(g, 0, stream_);
@@ -1873,6 +1930,7 @@ uint32_t Engine::hash_table_size(int32_t s) {
 }
 
 void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint32_t* si, uint32_t* nbr) {
+    join();
     Segment& sg = seg(s);
     ensure_committed();
     synchronize();
@@ -1897,6 +1955,7 @@ void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint
 }
 
 std::vector<int32_t> Engine::nonfinite_segments() {
+    join();
     ensure_committed();
     std::vector<int32_t> bad;
     if (table_seg_.empty()) return bad;

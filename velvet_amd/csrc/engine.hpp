@@ -175,6 +175,8 @@ public:
     size_t device_bytes_estimate(int32_t s);
     void simulate(float frame_dt);
     void synchronize();
+    // Make the solver's stream wait for the work the stream groups have queued.
+    void join();
     void set_use_graph(bool on);
     const Tuning& tuning() const { return tun_; }
     void set_tuning(const Tuning& t);
@@ -227,6 +229,7 @@ private:
     void pull_state(Segment& sg);
     void touch(int32_t s);                   // a structural change of segment s
     void invalidate_graph();
+    void sync_stream();                      // join(), then wait for the solver's stream
     void ensure_committed();
     void upload(Segment& sg, hipStream_t s);
     bool pools_fit(const Ranges& need);
@@ -234,9 +237,9 @@ private:
     void rebuild_tables();
     void upload_tables(const TableSet& t);
     void grow_pools(const Ranges& need);
-    void record_frame(float frame_dt, hipStream_t s);
+    void record_frame(float frame_dt, hipStream_t s, bool lazy);
     void hash_build(const DevState& d, int in, hipStream_t s);
-    void record_group(const DevState& d, float frame_dt, hipStream_t s);
+    void record_group(const DevState& d, float frame_dt, hipStream_t s, hipEvent_t after_hash = nullptr);
     void stage_begin(const char* label, hipStream_t s);
     void stage_end(hipStream_t s);
     void check_seg(int32_t s) const;
@@ -278,6 +281,7 @@ private:
     hipStream_t cstream_ = nullptr;   // uploads of staged segments
     hipEvent_t ev_fork_ = nullptr;
     std::vector<hipEvent_t> ev_join_;        // one per extra stream
+    size_t pending_join_ = 0;                // extra streams whose last frame the solver's stream has not joined
     std::vector<int32_t> table_seg_;         // table index -> segment id
     int passes_ = 1;
     int sorted_src_ = 0;

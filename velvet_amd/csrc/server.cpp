@@ -130,6 +130,7 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
         const hipStream_t es = e_.stream();
         VTX_HIP(hipEventRecord(ev0_, es));
         e_.simulate(frame_dt);
+        e_.join();                               // the frame's end is the end of every stream group
         VTX_HIP(hipEventRecord(ev1_, es));
         const auto s0 = clk::now();
         st.admitted += admit(true, st.admitted);   // overlaps the frame
