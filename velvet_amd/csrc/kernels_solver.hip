@@ -171,39 +171,44 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
         // neighbours four at a time: while a group's positions load, the next group's indices
         // load too; the four terms are then added in list order. An entry counts only if every
         // entry before it is a particle index (the list ends at the first sentinel, later rows are
-        // stale).
+        // stale). A neighbour's start-of-substep position enters only the friction of a contact,
+        // so it is loaded for contacts alone: most listed neighbours of a resting cloth lie
+        // beyond the diameter, and the pass is bound by the cache lines its gathers touch.
         bool live = true;
         uint32_t jn[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) jn[u] = (uint32_t)u < sg.max_nbr ? nb[(size_t)u * n] : VTX_INVALID;
         for (uint32_t k0 = 0; k0 < sg.max_nbr && live; k0 += 4) {
             bool ok[4];
+            uint32_t jg[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 live = live && jn[u] < n;
                 ok[u] = live;
+                jg[u] = sg.pbase + (ok[u] ? jn[u] : 0u);
             }
             float4 pj4[4], xj4[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t jg = sg.pbase + (ok[u] ? jn[u] : 0u);
-                pj4[u] = P[jg];
-                xj4[u] = d.X[jg];
-            }
+            for (int u = 0; u < 4; u++) pj4[u] = P[jg[u]];
 #pragma unroll
             for (int u = 0; u < 4; u++)
                 jn[u] = live && k0 + 4 + u < sg.max_nbr ? nb[(size_t)(k0 + 4 + u) * n] : VTX_INVALID;
+            bool hit[4];
+            float dist[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                if (!ok[u]) continue;
+                dist[u] = flen(p - xyz(pj4[u]));
+                hit[u] = ok[u] && !(w + pj4[u].w <= 0.f) && !(dist[u] >= sg.diameter);
+                if (hit[u]) xj4[u] = d.X[jg[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (!hit[u]) continue;
                 const float denom = w + pj4[u].w;
-                if (denom <= 0.f) continue;
                 const V3 pj = xyz(pj4[u]);
                 const V3 diff = p - pj;
-                const float dist = flen(diff);
-                if (dist >= sg.diameter) continue;
-                const V3 grad = diff * frcp(dist + VTX_EPSILON);
-                const float lambda = (dist - sg.diameter) * frcp(denom);
+                const V3 grad = diff * frcp(dist[u] + VTX_EPSILON);
+                const float lambda = (dist[u] - sg.diameter) * frcp(denom);
                 const V3 common = lambda * grad;
                 cnt++;
                 delta -= w * common;
