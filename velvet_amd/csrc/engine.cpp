@@ -1028,6 +1028,7 @@ void Engine::upload(Segment& sg, hipStream_t s) {
         const bool selfcol = sg.selfcol();
         d.M = selfcol ? sg.table_size() : 0;
         VTX_REQUIRE(!selfcol || d.M >= 1, VTX_EINVAL, "hash table size must be >= 1");
+        VTX_REQUIRE(!selfcol || d.M < (1u << 28), VTX_ECAPACITY, "hash table size must be < 2^28");
         d.tbase = (uint32_t)r.cell;
         d.nbase_lo = (uint32_t)(r.nbr & 0xFFFFFFFFu);
         d.nbase_hi = (uint32_t)((uint64_t)r.nbr >> 32);

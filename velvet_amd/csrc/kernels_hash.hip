@@ -400,6 +400,9 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
 // the rest. The walk is one flat loop over the cells' concatenation in the same order, with the
 // next candidate's position and initial position loaded while the current one is tested.
 #define VTX_NB_CACHE 16
+#ifndef VTX_NB_BATCH
+#define VTX_NB_BATCH 3   // cell ranges of the first sweep loaded per batch (27 = 9 batches of 3)
+#endif
 
 // The float4 at record i < 2^25 of the 32-B sorted records from a wave-uniform base (the
 // position half or, one float4 on, the initial-position half): a 32-bit byte offset lets the
@@ -443,14 +446,27 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const uint64_t mc = seg_mod(sg);
     const uint32_t K = sg.max_nbr;
     uint32_t mask = 0, total = 0, nz = 0;
+    // VTX_NB_BATCH cell ranges in flight at a time, each one 8-byte load at a 32-bit byte offset
+    // (tables have M < 2^28 cells)
 #pragma unroll
-    for (int q = 0; q < 27; q++) {
-        const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
-        const uint32_t len = c.x == VTX_INVALID ? 0u : min(c.y, c.x + K) - c.x;
-        if (len && nz < VTX_NB_CACHE) crange[nz][threadIdx.x] = c.x | len << 25;
-        nz += len ? 1u : 0u;
-        mask |= (len ? 1u : 0u) << q;
-        total += len;
+    for (int q0 = 0; q0 < 27; q0 += VTX_NB_BATCH) {
+        uint64_t cv[VTX_NB_BATCH];
+#pragma unroll
+        for (int r = 0; r < VTX_NB_BATCH; r++) {
+            const int q = q0 + r;
+            const uint32_t hq = hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc);
+            cv[r] = *reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(cells) + (hq << 3));
+        }
+#pragma unroll
+        for (int r = 0; r < VTX_NB_BATCH; r++) {
+            const int q = q0 + r;
+            const uint32_t cx = (uint32_t)cv[r], cy = (uint32_t)(cv[r] >> 32);
+            const uint32_t len = cx == VTX_INVALID ? 0u : min(cy, cx + K) - cx;
+            if (len && nz < VTX_NB_CACHE) crange[nz][threadIdx.x] = cx | len << 25;
+            nz += len ? 1u : 0u;
+            mask |= (len ? 1u : 0u) << q;
+            total += len;
+        }
     }
     if (total == 0) {
         nb[0] = VTX_INVALID;
