@@ -57,7 +57,7 @@ typedef struct vtx_params {
     float particle_diameter;         /* set by add_cloth (last cloth wins) */
     float particle_diameter_scalar;  /* 1.5 (used by mesh-side generators) */
     float hash_cell_size_scalar;     /* 1.5: cell = diameter * scalar */
-    int32_t hash_table_size;         /* extension: 0 = 2 x particles (reference) */
+    int32_t hash_table_size;         /* extension: 0 = 2 x particles (reference); < 2^28 */
     float stretch_stiffness;         /* extension: lambda multiplier; 1 = reference */
 } vtx_params;
 
