@@ -1554,12 +1554,19 @@ void Engine::record_frame(float frame_dt, hipStream_t s, bool lazy) {
         VTX_HIP(hipEventRecord(ev_fork_, s));
         record_group(groups_[0], frame_dt, s);
     }
-    for (size_t k = 1; k < groups_.size(); k++) {
-        VTX_HIP(hipStreamWaitEvent(xstreams_[k - 1], ev_fork_, 0));
-        record_group(groups_[k], frame_dt, xstreams_[k - 1]);
-        VTX_HIP(hipEventRecord(ev_join_[k - 1], xstreams_[k - 1]));
-    }
     pending_join_ = groups_.size() - 1;
+    try {
+        for (size_t k = 1; k < groups_.size(); k++) {
+            VTX_HIP(hipStreamWaitEvent(xstreams_[k - 1], ev_fork_, 0));
+            record_group(groups_[k], frame_dt, xstreams_[k - 1]);
+            VTX_HIP(hipEventRecord(ev_join_[k - 1], xstreams_[k - 1]));
+        }
+    } catch (...) {
+        // a launch failed part-way: whatever each stream holds is still joined later
+        if (lazy)
+            for (size_t k = 1; k < groups_.size(); k++) (void)hipEventRecord(ev_join_[k - 1], xstreams_[k - 1]);
+        throw;
+    }
     if (!lazy) join();
 }
 
