@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=30)
     ap.add_argument("--kernel-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--tune-frames", type=int, default=6,
+    ap.add_argument("--tune-frames", type=int, default=16,
                     help="frames per variant for the launch-variant tuner before warm-up; 0 = defaults")
     ap.add_argument("--set-tuning", default="", metavar="K=V,...",
                     help="fix launch variants instead of tuning, e.g. use_graph=1,neighbor_order=1")
