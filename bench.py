@@ -320,8 +320,9 @@ def main():
     ap.add_argument("--serve-budget-ms", type=float, default=11.0,
                     help="admission budget on the predicted frame time")
     ap.add_argument("--serve-graph", type=int, default=0)
-    ap.add_argument("--serve-tuning", default="", metavar="K=V,...",
-                    help="launch variants of the serving solver, e.g. streams=2")
+    ap.add_argument("--serve-tuning", default="neighbor_order=1,hash_lds=0,streams=2", metavar="K=V,...",
+                    help="launch variants of the serving solver (the live set changes every step, so it is "
+                         "not tuned online); the default is the measured best of a sweep on this job mix")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
