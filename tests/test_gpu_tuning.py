@@ -9,7 +9,8 @@
   under the tuned variant equals an untuned run bit for bit;
 * a second solver of the same shape class takes the cached tuning without timing;
 * host operations queued between frames (slot moves, colliders, particle edits, readback,
-  snapshot / restore) give the same results with 1 to 4 stream groups.
+  snapshot / restore) give the same results with 1 to 4 stream groups;
+* on a caller-bound stream, a copy queued right after a frame sees every stream group's work.
 """
 import sys
 
@@ -137,6 +138,42 @@ def test_host_operations_between_frames_with_stream_groups():
         assert np.array_equal(rb[0], rb_ref[0]) and np.array_equal(rb[1], rb_ref[1]), k
         for j, ((x, v), (xr, vr)) in enumerate(zip(got, ref)):
             assert np.array_equal(x, xr) and np.array_equal(v, vr), (k, j)
+
+
+def test_bound_stream_sees_every_group():
+    """On a stream the caller bound, work queued after vtx_simulate sees the whole frame, also
+    the segments of the extra stream groups (no lazy join there)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(stream)) == 0
+    scenes = _batch()
+    try:
+        with ClothSolver(use_graph=False) as s:
+            segs = [s.load_scene(sc) for sc in scenes]
+            s.set_tuning(streams=2)
+            s.set_stream(stream.value)
+            last = segs[-1]                      # in the second group
+            ptr, n = s.device_positions(last)    # uploads; the address stays valid (no registration follows)
+            host = C.c_void_p()
+            assert hip.hipHostMalloc(C.byref(host), C.c_size_t(16 * n), 0) == 0
+            try:
+                got = []
+                for f in range(3):
+                    s.Simulate()
+                    # a copy queued on the bound stream right after the frame, no vtx_synchronize
+                    assert hip.hipMemcpyAsync(host, C.c_void_p(ptr), C.c_size_t(16 * n), 2, stream) == 0
+                    assert hip.hipStreamSynchronize(stream) == 0
+                    got.append(np.ctypeslib.as_array(C.cast(host, C.POINTER(C.c_float)), shape=(n, 4))[:, :3].copy())
+                    s.synchronize()
+                    assert np.array_equal(got[-1], s.positions(last)), f
+            finally:
+                hip.hipHostFree(host)
+            s.set_stream(None)
+    finally:
+        hip.hipStreamDestroy(stream)
+    ref = _run(scenes, 3)
+    assert np.array_equal(got[-1], ref[-1][0])
 
 
 def test_autotune_keeps_state_and_caches():
