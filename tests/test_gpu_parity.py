@@ -91,18 +91,18 @@ def test_hash_stage_bit_exact_packed_segments():
     s.close()
 
 
-@pytest.mark.parametrize("hash_lds", [1, 0])
-def test_hash_stage_bit_exact_mixed_paths(hash_lds):
+@pytest.mark.parametrize("hash_lds,order", [(1, 2), (0, 2), (1, 1), (0, 1), (1, 0)])
+def test_hash_stage_bit_exact_mixed_paths(hash_lds, order):
     """Segments hashed in one workgroup each (at most 16384 particles, at most 2^16 cells; 128 x 128
     is the largest) packed with segments that take the device-wide sort (131 x 131 = 17161
     particles; a small cloth with a 2^17-cell table): every segment's tables and lists match the
-    oracle, with the small-segment path on or off."""
-    say(f"hash mixed paths hash_lds={hash_lds}")
+    oracle, with the small-segment path on or off and every neighbour-search thread order."""
+    say(f"hash mixed paths hash_lds={hash_lds} neighbor_order={order}")
     items = [_folded(16, seed=1), _folded(131, seed=2), _folded(40, seed=3), _folded(20, seed=4),
              _folded(128, seed=6)]                  # 16384 particles: the largest one-workgroup build
     sizes = [0, 0, 1 << 17, 97, 0]
     s = _hash_solver(items, sizes)
-    s.set_tuning(hash_lds=hash_lds)
+    s.set_tuning(hash_lds=hash_lds, neighbor_order=order)
     for k, (c, p) in enumerate(items):
         s.write_predicted(p, seg=k)
     s.run_hash()
