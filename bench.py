@@ -328,8 +328,8 @@ def main():
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the batched "
-                                   "and served jobs shard across GPUs with no exchange)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (batched and "
+                                   "served jobs would shard across GPUs with no exchange)"}))
         sys.exit(2)
 
     scenes, desc = workload_scenes(args.workload)
