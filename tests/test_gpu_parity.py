@@ -114,6 +114,29 @@ def test_hash_stage_bit_exact_mixed_paths(hash_lds, order):
     s.close()
 
 
+def test_hash_table_size_limit():
+    """Cell tables of 2^28 or more cells are refused before anything is allocated (the neighbour
+    search addresses a table with 32-bit byte offsets); 2^28 - 1 is accepted."""
+    from velvet_amd._lib import VtxError
+    c, p = _folded(8, seed=1)
+    for M, ok in ((1 << 28, False), ((1 << 28) - 1, True)):
+        say(f"hash table size {M}")
+        s = ClothSolver(use_graph=False)
+        try:
+            prm = S.Params(hash_table_size=M, particle_diameter=float(c.diameter))
+            s.set_params(prm, 0)
+            s.AddCloth(c.x, c.tris, c.diameter)
+            s.set_params(prm, 0)
+            if ok:
+                s.Simulate()
+                s.synchronize()
+            else:
+                with pytest.raises(VtxError, match="2\\^28"):
+                    s.Simulate()
+        finally:
+            s.close()
+
+
 @pytest.mark.parametrize("shift", [117966.5, -117966.2])
 def test_hash_stage_bit_exact_far_from_origin(shift):
     """A folded cloth about 2^20 cells from the origin on x (positions with a 1/128 m binary32

@@ -726,6 +726,7 @@ void Engine::prepare(int32_t si) {
     const uint32_t n = sg.n();
     VTX_REQUIRE(!sg.selfcol() || n < (1u << 25), VTX_ECAPACITY,
                 "self collision supports at most 2^25 particles per segment");
+    VTX_REQUIRE(!sg.selfcol() || sg.table_size() < (1u << 28), VTX_ECAPACITY, "hash table size must be < 2^28");
     fill_state(sg, pr);
     const uint32_t L = (uint32_t)(sg.slots.size() / 3);
     for (auto& c : sg.cols) pr.cols.push_back(pack_collider(c));
