@@ -676,9 +676,9 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
 }
 
 // ComputeTriangleNormals + ComputeVertexNormals (Velvet/VtClothSolverGPU.cu:419-450) as a
-// per-vertex gather over incident triangles.
-__global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d) {
-    const TileDev t = d.tiles[tile_of(d.remap)];
+// per-vertex gather over incident triangles (segments off the grid path).
+__global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d, const TileDev* __restrict__ tiles) {
+    const TileDev t = tiles[tile_of(d.remap)];
     const uint32_t lane = threadIdx.x;
     if (lane >= t.count) return;
     const SegDev sg = d.segs[t.seg];
@@ -694,6 +694,52 @@ __global__ __launch_bounds__(VTX_TILE) void k_normals(DevState d) {
         s += cross(b - a, c - a);
     }
     d.Nrm[g] = f4(normalize(s), 0.f);
+}
+
+// The same for a grid segment's 16 x 16 tile: positions of the tile and a one-vertex halo staged
+// in LDS, one read per vertex instead of the incidence gather. Quad (a, b) holds triangles
+// (A, B, C) and (C, B, D) with A = (a, b), B = (a+1, b), C = (a, b+1), D = (a+1, b+1) (the grid
+// the stencil path accepts), so a vertex's incident triangles in triangle-index order are T1 of
+// quad (r-1, c-1), T0 and T1 of (r-1, c), T0 and T1 of (r, c-1), T0 of (r, c), each crossed as
+// cross(b - a, c - a) in its stored vertex order: the terms and the order of the general gather.
+__global__ __launch_bounds__(VTX_TILE) void k_normals_grid(DevState d, const GridTileDev* __restrict__ tiles) {
+    constexpr int GS = VTX_GT + 2;                 // staged side: tile + one-vertex halo
+    __shared__ float4 sx[GS * GS];
+    const GridTileDev t = tiles[tile_of(d.remap)];
+    const SegDev sg = d.segs[t.seg];
+    const int rows = (int)sg.rows, cols = (int)sg.cols, a0 = (int)t.a0, b0 = (int)t.b0;
+    for (uint32_t e = threadIdx.x; e < GS * GS; e += VTX_TILE) {
+        const int a = a0 - 1 + (int)(e / GS), b = b0 - 1 + (int)(e % GS);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (a >= 0 && a < rows && b >= 0 && b < cols) v = d.X[sg.pbase + (uint32_t)(a * cols + b)];
+        sx[e] = v;
+    }
+    __syncthreads();
+    const int ty = (int)(threadIdx.x / VTX_GT), tx = (int)(threadIdx.x % VTX_GT);
+    const int r = a0 + ty, c = b0 + tx;
+    if (r >= rows || c >= cols) return;
+    auto at = [&](int dr, int dc) { return xyz(sx[(ty + 1 + dr) * GS + (tx + 1 + dc)]); };
+    const V3 v = at(0, 0);
+    V3 s = v3(0.f, 0.f, 0.f);
+    if (r >= 1 && c >= 1) {                        // quad (r-1, c-1): v = D
+        const V3 B = at(0, -1), C = at(-1, 0);
+        s += cross(B - C, v - C);
+    }
+    if (r >= 1 && c + 1 < cols) {                  // quad (r-1, c): v = B
+        const V3 A = at(-1, 0), C = at(-1, 1), D = at(0, 1);
+        s += cross(v - A, C - A);
+        s += cross(v - C, D - C);
+    }
+    if (r + 1 < rows && c >= 1) {                  // quad (r, c-1): v = C
+        const V3 A = at(0, -1), B = at(1, -1), D = at(1, 0);
+        s += cross(B - A, v - A);
+        s += cross(B - v, D - v);
+    }
+    if (r + 1 < rows && c + 1 < cols) {            // quad (r, c): v = A
+        const V3 B = at(1, 0), C = at(0, 1);
+        s += cross(B - v, C - v);
+    }
+    d.Nrm[sg.pbase + (uint32_t)(r * cols + c)] = f4(normalize(s), 0.f);
 }
 
 // Failure detection: flags[t.seg] = 1 when a particle of the segment has a non-finite position
@@ -778,8 +824,9 @@ hipError_t launch_check_finite(const DevState& d, uint32_t* flags, uint32_t nseg
 }
 
 hipError_t launch_normals(const DevState& d, hipStream_t s) {
-    if (!d.ntiles) return hipSuccess;
-    hipLaunchKernelGGL(k_normals, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d);
+    if (d.nitiles) hipLaunchKernelGGL(k_normals, dim3(d.nitiles), dim3(VTX_TILE), 0, s, d, d.itiles);
+    if (d.ngtiles) hipLaunchKernelGGL(k_normals_grid, dim3(d.ngtiles), dim3(VTX_TILE), 0, s, d, d.gtiles);
+    if (d.nltiles) hipLaunchKernelGGL(k_normals_grid, dim3(d.nltiles), dim3(VTX_TILE), 0, s, d, d.ltiles);
     return last_launch();
 }
 
