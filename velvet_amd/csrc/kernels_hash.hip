@@ -99,7 +99,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
     return off + incl - v;
 }
 
-// Device-wide exclusive scan of hist[0..n): block sums, scan of sums, apply.
+// Device-wide exclusive scan of hist[0..n) in two launches: block sums, then each block adds
+// the sums of the blocks before it (a block reduction over at most a few hundred sums) and
+// scans its own elements.
 __global__ __launch_bounds__(VTX_TILE) void k_scan_reduce(uint32_t* __restrict__ a, uint32_t n,
                                                           uint32_t* __restrict__ part) {
     __shared__ uint32_t sh[VTX_TILE / 64];
@@ -114,24 +116,17 @@ __global__ __launch_bounds__(VTX_TILE) void k_scan_reduce(uint32_t* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(VTX_TILE) void k_scan_partials(uint32_t* __restrict__ part, uint32_t nb) {
-    __shared__ uint32_t sh[VTX_TILE / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += VTX_TILE) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? part[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_excl_scan(v, &total, sh);
-        if (i < nb) part[i] = carry + ex;
-        carry += total;
-    }
-}
-
 __global__ __launch_bounds__(VTX_TILE) void k_scan_apply(uint32_t* __restrict__ a, uint32_t n,
                                                          const uint32_t* __restrict__ part) {
     __shared__ uint32_t sh[VTX_TILE / 64];
     const uint32_t base = blockIdx.x * VTX_SCAN_BLOCK;
-    uint32_t carry = part[blockIdx.x];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < blockIdx.x; b0 += VTX_TILE) {   // sums of the earlier blocks
+        const uint32_t b = b0 + threadIdx.x;
+        uint32_t total;
+        block_excl_scan(b < blockIdx.x ? part[b] : 0u, &total, sh);
+        carry += total;
+    }
     for (uint32_t r = 0; r < VTX_SCAN_BLOCK / VTX_TILE; r++) {
         const uint32_t i = base + r * VTX_TILE + threadIdx.x;
         const uint32_t v = i < n ? a[i] : 0u;
@@ -537,7 +532,6 @@ hipError_t launch_scan(uint32_t* a, uint32_t n, uint32_t* part, hipStream_t s) {
     const uint32_t nb = (n + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK;
     if (!nb) return hipSuccess;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(VTX_TILE), 0, s, a, n, part);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(VTX_TILE), 0, s, part, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(VTX_TILE), 0, s, a, n, (const uint32_t*)part);
     return hipGetLastError();
 }
