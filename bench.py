@@ -361,8 +361,9 @@ def main():
 
     lat = latency(solver, args.latency_steps)
     kt = kernel_times(solver, args.kernel_steps)
-    it = kt.get("k_iterate", {"ms": float("nan"), "launches": 1})
-    avg_ms = it["ms"] / max(1, it["launches"])
+    it_ms = sum(v["ms"] for v in kt.values() if v.get("kernel") == "k_iterate")
+    it_n = sum(v["launches"] for v in kt.values() if v.get("kernel") == "k_iterate")
+    avg_ms = it_ms / it_n if it_n else float("nan")
     alg_bytes = S.algorithmic_bytes_per_iteration(tot)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(args.workload)

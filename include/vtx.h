@@ -220,10 +220,12 @@ int vtx_restore(vtx_solver* s, int32_t seg, const void* buf, uint64_t bytes);
 
 /* Extension replacing the reference's CUDA-GL interop (Velvet/VtBuffer.hpp:167-229) for a viewer
  * that runs at its own pace: vtx_readback_request queues a copy of a segment's positions
- * (float4, w = inverse mass) and normals (float4) into the older of two pinned host buffers,
- * after the work queued so far; vtx_readback_latest returns the newest completed copy without
- * waiting (*positions = NULL if none has completed) and the number of frames simulated before it
- * was requested. A returned buffer stays valid until the request after next. */
+ * (float4, w = inverse mass) and normals (float4) into one of three pinned host buffers (the
+ * least recently requested one that vtx_readback_latest did not hand out last), after the work
+ * queued so far; vtx_readback_latest returns the newest completed copy without waiting
+ * (*positions = NULL if none has completed) and the number of frames simulated before it was
+ * requested. A returned buffer stays valid until the next vtx_readback_latest call on the
+ * segment: no request writes into it before then. */
 int vtx_readback_request(vtx_solver* s, int32_t seg);
 int vtx_readback_latest(vtx_solver* s, int32_t seg, const float** positions, const float** normals, uint32_t* n,
                         uint64_t* frame);
@@ -246,11 +248,13 @@ int vtx_hash_table_size(vtx_solver* s, int32_t seg, uint32_t* M);
 int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* cell_end,
                   uint32_t* sorted_hash, uint32_t* sorted_index, uint32_t* nbr);
 
-/* Per-kernel GPU time, the counterpart of the reference's ScopedTimerGPU labels
- * (Velvet/Timer.hpp:242-260, Velvet/GUI.cpp:32-51). With timing on, frames are launched
- * without the graph and every launch group is bracketed by a HIP event pair on the solver's
- * stream; vtx_read_timing returns JSON {label: {"ms": total, "launches": count}} accumulated
- * since timing was switched on. */
+/* Per-stage GPU time under the reference's ScopedTimerGPU labels (Velvet/Timer.hpp:242-260,
+ * Velvet/GUI.cpp:32-51). With timing on, frames are launched without the graph and every launch
+ * group is bracketed by a HIP event pair on the solver's stream; vtx_read_timing returns JSON
+ * {label: {"ms": total, "launches": count, "kernel": name}} accumulated since timing was switched
+ * on, plus "total". A label is the reference's stage label ("Solver_HashCache", ...), or several
+ * joined by '+' where one kernel runs those stages fused ("Solver_CollideParticles+
+ * Solver_ApplyDeltas+Solver_CollideSDFs" for k_collide); "kernel" names the kernel(s). */
 int vtx_set_timing(vtx_solver* s, int32_t on);
 int vtx_read_timing(vtx_solver* s, char* buf, size_t len);
 

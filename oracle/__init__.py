@@ -70,6 +70,9 @@ def lib():
         L.cpu_solver_simulate.argtypes = [C.c_void_p, f32]
         L.cpu_solver_positions.argtypes = [C.c_void_p, P(f32)]
         L.cpu_solver_destroy.argtypes = [C.c_void_p]
+        L.cpu_solver_set_state.argtypes = [C.c_void_p, P(f32), P(f32)]
+        L.cpu_solver_velocities.argtypes = [C.c_void_p, P(f32)]
+        L.cpu_solver_neighbor_counts.argtypes = [C.c_void_p, P(i32)]
         _LIB = L
     return _LIB
 
@@ -238,6 +241,22 @@ class CpuSolver:
     def positions(self):
         out = np.empty((self.n, 3), np.float32)
         lib().cpu_solver_positions(self.h, _p(out, f32))
+        return out
+
+    def velocities(self):
+        out = np.empty((self.n, 3), np.float32)
+        lib().cpu_solver_velocities(self.h, _p(out, f32))
+        return out
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        lib().cpu_solver_set_state(self.h, _p(xa, f32) if xa is not None else None,
+                                   _p(va, f32) if va is not None else None)
+
+    def neighbor_counts(self):
+        out = np.empty(self.n, np.int32)
+        lib().cpu_solver_neighbor_counts(self.h, _p(out, i32))
         return out
 
     def __del__(self):

@@ -351,6 +351,28 @@ void cpu_solver_positions(const cpu_solver* s, float* out) {
     }
 }
 
+/* Test access: replace positions and velocities (n xyz triples each; NULL keeps them), read the
+ * velocities, and read each particle's neighbour-list length from the frame's hash (every entry
+ * of the 27 probed cells, itself and duplicates included, Velvet/SpatialHashCPU.hpp:96-123). */
+void cpu_solver_set_state(cpu_solver* s, const float* x, const float* v) {
+    for (int32_t i = 0; i < s->n; i++) {
+        if (x) s->x[i] = F(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        if (v) s->v[i] = F(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+    }
+}
+
+void cpu_solver_velocities(const cpu_solver* s, float* out) {
+    for (int32_t i = 0; i < s->n; i++) {
+        out[3 * i] = s->v[i].x;
+        out[3 * i + 1] = s->v[i].y;
+        out[3 * i + 2] = s->v[i].z;
+    }
+}
+
+void cpu_solver_neighbor_counts(const cpu_solver* s, int32_t* out) {
+    for (int32_t i = 0; i < s->n; i++) out[i] = s->nb_n[i];
+}
+
 void cpu_solver_destroy(cpu_solver* s) {
     if (!s) return;
     for (int32_t i = 0; i < s->n; i++) free(s->nb[i]);

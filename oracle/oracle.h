@@ -121,6 +121,9 @@ cpu_solver* cpu_solver_create(int32_t resolution_plus_one, const float* x, int32
                               float friction, float margin, float bend_compliance);
 void cpu_solver_simulate(cpu_solver* s, float frame_dt);
 void cpu_solver_positions(const cpu_solver* s, float* out);
+void cpu_solver_set_state(cpu_solver* s, const float* x, const float* v);
+void cpu_solver_velocities(const cpu_solver* s, float* out);
+void cpu_solver_neighbor_counts(const cpu_solver* s, int32_t* out);
 void cpu_solver_destroy(cpu_solver* s);
 
 #ifdef __cplusplus

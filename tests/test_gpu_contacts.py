@@ -1,0 +1,285 @@
+"""Contact-heavy and kinematic-collider scenes on the HIP path against the FP64 oracle.
+
+* particle contacts (Velvet/VtClothSolverGPU.cu:329-386): two stacked cloths in one segment (the
+  reference's shared solver, general Jacobi path) and one cloth folded onto itself (grid stencil
+  path), the upper layer moving into the lower one so that every particle is in contact from the
+  first substep; within 1e-5 x extent after that substep and 1e-3 x extent after 10 frames;
+* moving colliders (VelocityAt, Velvet/VtClothSolverGPU.cuh:91-96) updated every frame, the cube
+  SDF (cuh:44-87) rotated, scaled and spinning, with points in its rounded-corner band;
+* AddCloth with a model matrix (InitializePositions, Velvet/VtClothSolverGPU.cu:30-40).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from velvet_amd import scenes as S
+from velvet_amd.solver import ClothSolver
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _with_positions(c, x):
+    """The cloth c moved to positions x, with stretch rest lengths measured there."""
+    x = np.ascontiguousarray(x, dtype=F32)
+    rest = S._length_f32(x[c.stretch[:, 0]] - x[c.stretch[:, 1]])
+    return S.Cloth(**{**c.__dict__, "x": x, "stretch_rest": rest})
+
+
+def stacked(n=24, spacing=0.05, gap=1.05):
+    """Two cloths in one segment: one lying on a plane (at the collision margin), one gap x d above."""
+    lo = S.grid_cloth(n, spacing, center=(0.0, 0.06, 0.0), jitter=1e-4, seed=0, stream=21)
+    d = float(lo.diameter)
+    hi = S.grid_cloth(n, spacing, center=(0.0, 0.06 + gap * d, 0.0), jitter=1e-4, seed=0, stream=22)
+    p = S.Params(num_substeps=5, num_iterations=10)
+    sc = S.Scene(S.finish_params(p, [lo, hi]), [lo, hi], [S.plane(0.0)], 10, "stacked")
+    return sc, np.arange(lo.n, lo.n + hi.n)
+
+
+def folded(n=24, spacing=0.05, gap=1.05):
+    """One grid cloth folded onto itself: rows beyond the middle lie gap x d above the first half,
+    running back (a z-fold), with rest lengths of the folded shape."""
+    c = S.grid_cloth(n, spacing, center=(0.0, 0.06, 0.0), jitter=1e-4, seed=0, stream=23)
+    d = float(c.diameter)
+    x = c.x.astype(np.float64).copy()
+    k = n // 2
+    rows = np.arange(n * n) // n
+    z0 = x[:, 2].min()
+    up = rows > k
+    x[up, 2] = x[up, 2] - 2 * (rows[up] - k) * spacing
+    x[up, 1] += gap * d
+    c2 = _with_positions(c, x)
+    p = S.Params(num_substeps=5, num_iterations=10)
+    sc = S.Scene(S.finish_params(p, [c2]), [c2], [S.plane(0.0)], 10, "folded")
+    assert z0 <= x[:, 2].min() + 1e-6
+    return sc, np.nonzero(up)[0]
+
+
+def _contact_pairs(sc, x, v, h):
+    """Pairs within the diameter among the oracle's neighbour lists at the first substep."""
+    g = np.asarray(sc.params.gravity, np.float64)
+    p = x + (v + g * h) * h
+    d = float(sc.params.particle_diameter)
+    cs, ce, si, nb = oracle.hash64(p, x, float(S.hash_cell_size(sc.params)), d, 2 * sc.n)
+    pairs, inv = 0, set()
+    for i, lst in enumerate(oracle.neighbor_lists(nb)):
+        for j in lst:
+            if np.linalg.norm(p[i] - p[j]) < d:
+                pairs += 1
+                inv.add(i)
+    return pairs, len(inv)
+
+
+def _run_pair(sc, moving, vdown, frames, frame_dt=None, stencil_expected=None):
+    """GPU and FP64 runs of sc with the particles in `moving` given velocity (0, vdown, 0)."""
+    with ClothSolver() as s:
+        s.load_scene(sc)
+        if stencil_expected is not None:
+            assert s.stencil_segments() == stencil_expected
+        vel = np.array([0.0, vdown, 0.0], F32)
+        for i in moving:
+            s.set_particle(int(i), vel=vel)
+        for _ in range(frames):
+            s.Simulate(frame_dt)
+        s.synchronize()
+        xg = s.positions().astype(np.float64)
+    sim = oracle.OracleSim(sc, frame_dt)
+    sim.v.reshape(-1, 3)[moving, 1] = float(F32(vdown))
+    sim.run(frames)
+    return xg, sim.x.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("kind", ["stacked", "folded"])
+def test_contacts_one_substep_vs_fp64(kind):
+    sc, moving = stacked() if kind == "stacked" else folded()
+    sc.params.num_substeps = 1
+    h = F32(S.FRAME_DT / F32(5))
+    d = float(sc.params.particle_diameter)
+    vdown = -0.07 * d / float(h)
+    x0 = S.flatten(sc)["x"].astype(np.float64)
+    v0 = np.zeros_like(x0)
+    v0[moving, 1] = float(F32(vdown))
+    pairs, parts = _contact_pairs(sc, x0, v0, float(h))
+    say(f"contacts {kind}: {pairs} contact pairs, {parts} of {sc.n} particles in contact at substep 0")
+    assert parts >= 0.1 * sc.n and pairs >= 500
+    xg, xr = _run_pair(sc, moving, vdown, 1, frame_dt=h, stencil_expected=1 if kind == "folded" else 0)
+    err = float(np.max(np.abs(xg - xr)))
+    say(f"  max err {err:.3e} (extent {sc.extent():.3f})")
+    assert err <= 1e-5 * sc.extent()
+    assert float(np.max(np.abs(xr - x0 - v0 * float(h)))) > 1e-4   # the contacts moved the cloth
+
+
+@pytest.mark.parametrize("kind", ["stacked", "folded"])
+def test_contacts_ten_frames_vs_fp64(kind):
+    sc, moving = stacked() if kind == "stacked" else folded()
+    d = float(sc.params.particle_diameter)
+    h = float(S.FRAME_DT) / sc.params.num_substeps
+    vdown = -0.07 * d / h
+    xg, xr = _run_pair(sc, moving, vdown, 10)
+    err = float(np.max(np.abs(xg - xr)))
+    say(f"contacts {kind} 10 frames: max err {err:.3e} (extent {sc.extent():.3f})")
+    assert np.all(np.isfinite(xg))
+    assert err <= 1e-3 * sc.extent()
+    # the upper layer rests on the lower one: no particle passed through
+    assert float(np.min(xr[moving, 1])) > 0.06 + 0.5 * d
+
+
+# ------------------------------------------------------------------ kinematic colliders
+def _run_animated(sc, collider_at, frames):
+    """GPU and FP64 runs with colliders re-registered every frame (collider_at(f) = the list for
+    frame f, each with its previous frame's transform as `last`)."""
+    with ClothSolver() as s:
+        s.load_scene(sc)
+        for f in range(frames):
+            s.UpdateColliders(collider_at(f))
+            s.Simulate()
+        s.synchronize()
+        xg = s.positions().astype(np.float64)
+    sim = oracle.OracleSim(sc)
+    for f in range(frames):
+        sim.cols = oracle.make_colliders(collider_at(f))
+        sim.sc.cols = sim.cols
+        sim.sc.C = len(collider_at(f))
+        sim.run(1)
+    return xg, sim.x.reshape(-1, 3).copy()
+
+
+def _cloth_scene(n=24, spacing=0.05, y=1.0, attached=(), frames=10, **params):
+    c = S.grid_cloth(n, spacing, center=(0.0, y, 0.0), jitter=1e-4, seed=3, stream=31, attached=attached)
+    p = S.Params(num_substeps=5, num_iterations=10, **params)
+    return S.Scene(S.finish_params(p, [c]), [c], [], frames, "animated")
+
+
+def test_moving_sphere_vs_fp64():
+    """A sphere rising into a cloth pinned at two corners while sliding sideways: the friction of
+    every contact uses the sphere's velocity from its last and current transforms."""
+    say("moving sphere vs fp64")
+    sc = _cloth_scene(attached=(0, 23), friction=0.4)
+    dt = float(S.FRAME_DT)
+    vel = np.array([0.6, 0.9, -0.3])
+    start = np.array([0.0, 0.35, 0.0])
+
+    def at(f):
+        cur = start + vel * dt * (f + 1)
+        last = start + vel * dt * f
+        return [S.Collider(S.SPHERE, tuple(cur), (0.3, 0.3, 0.3), last_position=tuple(last))]
+
+    frames = 12
+    xg, xr = _run_animated(sc, at, frames)
+    err = float(np.max(np.abs(xg - xr)))
+    # same run with a static sphere at each frame's place: the sphere's motion matters
+    still = lambda f: [S.Collider(S.SPHERE, tuple(start + vel * dt * (f + 1)), (0.3, 0.3, 0.3))]
+    _, xs = _run_animated(sc, still, frames)
+    say(f"  max err {err:.3e}, moving vs static sphere differ by {np.max(np.abs(xs - xr)):.3e}")
+    assert err <= 1e-3 * sc.extent()
+    assert float(np.max(np.abs(xs - xr))) > 10 * max(err, 1e-6)
+
+
+def test_spinning_scaled_cube_vs_fp64():
+    """A cloth falling onto a non-uniformly scaled cube that spins about y (and is tilted), and a
+    second cloth corner dropped onto a cube's corner (rounded-corner band of the cube SDF)."""
+    say("spinning cube vs fp64")
+    sc = _cloth_scene(y=0.55, friction=0.3, frames=15)
+    dt = float(S.FRAME_DT)
+    spin = 120.0          # degrees per second about y
+
+    def at(f):
+        return [S.Collider(S.CUBE, (0.05, 0.2, -0.03), (0.7, 0.4, 0.5), rotation=(8.0, spin * dt * (f + 1), 5.0),
+                           last_rotation=(8.0, spin * dt * f, 5.0))]
+
+    frames = 15
+    xg, xr = _run_animated(sc, at, frames)
+    err = float(np.max(np.abs(xg - xr)))
+    say(f"  spinning cube: max err {err:.3e}")
+    assert err <= 1e-3 * sc.extent()
+    mid = 12 * 24 + 12
+    assert xr[mid, 1] > 0.4 + 0.05                # the cube's top face (+ margin) holds the cloth up
+
+
+def test_cube_corner_band_vs_fp64():
+    """Particles started inside the rounded-corner and edge bands of a rotated cube are moved out
+    a fifth of the way per SDF pass (scalar 0.2): GPU and FP64 agree on every pass."""
+    say("cube corner band vs fp64")
+    margin = 0.06
+    s_ = 0.8
+    half = 0.5 + margin / s_
+    rng = np.random.default_rng(11)
+    # local points: corner band (all three offsets in (-0.03, 0)), edge band (two), face
+    pts = []
+    for k in range(64):
+        sg = np.where(rng.random(3) < 0.5, -1.0, 1.0)
+        off = -rng.uniform(0.012, 0.028, 3)
+        if k % 3 == 1:
+            off[k % 3] = -rng.uniform(0.1, 0.3)
+        elif k % 3 == 2:
+            off[:2] = -rng.uniform(0.1, 0.3, 2)
+        pts.append(sg * (half + off))
+    rot = (25.0, 40.0, -15.0)
+    M = S.transform_matrix((0.1, 0.3, -0.2), rot, (s_, s_, s_))
+    world = (M[:3, :3] @ np.asarray(pts).T).T + M[:3, 3]
+    c = S.Cloth(x=world.astype(F32), tris=np.zeros((0, 3), np.uint32), stretch=np.zeros((0, 2), np.int32),
+                stretch_rest=np.zeros(0, F32), bend=np.zeros((0, 4), np.uint32), bend_rest=np.zeros(0, F32),
+                slots=np.zeros((0, 3), F32), attach_pid=np.zeros(0, np.int32), attach_slot=np.zeros(0, np.int32),
+                attach_dist=np.zeros(0, F32), diameter=F32(0.01))
+    p = S.Params(num_substeps=3, num_iterations=2, gravity=(0.0, 0.0, 0.0), enable_self_collision=False,
+                 collision_margin=margin, friction=0.2)
+    col = S.Collider(S.CUBE, (0.1, 0.3, -0.2), (s_, s_, s_), rotation=rot)
+    sc = S.Scene(S.finish_params(p, [c]), [c], [col], 2, "cube-bands")
+    with ClothSolver() as s:
+        s.load_scene(sc)
+        for _ in range(2):
+            s.Simulate()
+        s.synchronize()
+        xg = s.positions().astype(np.float64)
+    xr = oracle.simulate(sc, frames=2).x.reshape(-1, 3)
+    moved = np.linalg.norm(xr - world, axis=1)
+    err = float(np.max(np.abs(xg - xr)))
+    say(f"  max err {err:.3e}; moved {int(np.sum(moved > 1e-5))} of {len(pts)} points")
+    assert err <= 1e-6
+    assert np.sum(moved > 1e-5) >= 40
+
+
+def test_add_cloth_with_model_matrix():
+    """AddCloth transforms positions by the model matrix: the device holds the host-transformed
+    positions (to rounding), and a frame from there matches the FP64 oracle started from them."""
+    say("model matrix")
+    base = S.grid_cloth(20, 0.05, center=(0.0, 0.0, 0.0), jitter=1e-4, seed=2, stream=41, attached=(0, 19))
+    M = S.transform_matrix((0.3, 1.2, -0.4), (20.0, 35.0, -10.0), (1.0, 1.0, 1.0))
+    world = ((M[:3, :3] @ base.x.astype(np.float64).T).T + M[:3, 3]).astype(F32)
+    cw = _with_positions(base, world)
+    slots = world[[0, 19]]
+    dist = np.concatenate([S._length_f32(slots[k][None, :] - world) for k in range(2)]).astype(F32)
+    cw = S.Cloth(**{**cw.__dict__, "slots": slots, "attach_dist": dist})
+    p = S.Params(num_substeps=5, num_iterations=10)
+    sc = S.Scene(S.finish_params(p, [cw]), [cw], [S.sphere((0.3, 0.6, -0.4), 0.3)], 5, "model")
+    with ClothSolver() as s:
+        s.set_params(sc.params)
+        off = s.AddCloth(base.x, base.tris, base.diameter, model=M)
+        x_dev = s.positions()
+        s.AddStretchN(cw.stretch + off, cw.stretch_rest)
+        for k in range(2):
+            s.AddAttachSlot(slots[k])
+        s.AddAttachN(cw.attach_pid, cw.attach_slot, cw.attach_dist)
+        s.AddBendN(cw.bend, cw.bend_rest)
+        s.set_params(sc.params)
+        s.UpdateColliders(sc.colliders)
+        np.testing.assert_allclose(x_dev, world, atol=4e-7)
+        for _ in range(5):
+            s.Simulate()
+        s.synchronize()
+        xg = s.positions().astype(np.float64)
+    xr = oracle.simulate(sc, frames=5).x.reshape(-1, 3)
+    err = float(np.max(np.abs(xg - xr)))
+    say(f"  device vs host transform {np.max(np.abs(x_dev - world)):.2e}, 5 frames max err {err:.3e}")
+    assert err <= 1e-3 * sc.extent()
+    # a flat 16-vector is column-major (glm memory): the same transform
+    with ClothSolver() as s:
+        s.AddCloth(base.x, base.tris, base.diameter, model=M.T.reshape(16))
+        assert np.array_equal(s.positions(), x_dev)

@@ -431,6 +431,11 @@ def test_snapshot_restore_roundtrip():
             s.Simulate()
         s.synchronize()
         assert np.array_equal(s.positions(2), xa) and np.array_equal(s.velocities(2), va)
+        # the replayed state is the simulation's: the FP64 oracle after 6 frames
+        ref = oracle.simulate(scenes[2], frames=6)
+        err = float(np.max(np.abs(xa.astype(np.float64) - ref.x.reshape(-1, 3))))
+        say(f"  restored run vs fp64 after 6 frames: {err:.3e}")
+        assert err <= 1e-3 * scenes[2].extent()
         with pytest.raises(Exception):
             s.restore(snap[:-16], seg=2)
         if scenes[1].n != scenes[2].n:
@@ -509,9 +514,10 @@ def test_add_mesh_constraints_matches_registered_arrays():
         assert np.array_equal(s.positions(), xa) and np.array_equal(s.velocities(), va)
 
 
-def test_viewer_readback_double_buffer():
+def test_viewer_readback_buffers():
     """Readbacks queued after frames return the positions and normals of exactly that frame, the
-    newest completed one first, with the frame count; nothing before the first request."""
+    newest completed one first, with the frame count; nothing before the first request; the
+    buffer handed out is not written by later requests."""
     say("viewer readback")
     sc = S.config1(frames=4)
     with ClothSolver() as s:
@@ -534,6 +540,26 @@ def test_viewer_readback_double_buffer():
         s.synchronize()
         pos, _, frame = s.readback_latest()
         assert frame == 4 and np.array_equal(pos[:, :3], s.positions())
+        # the buffer handed out stays untouched by later requests until the next readback_latest
+        import ctypes as C
+        from velvet_amd._lib import lib
+        pp, qq = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+        nn, ff = C.c_uint32(), C.c_uint64()
+        assert lib().vtx_readback_latest(s.h, 0, C.byref(pp), C.byref(qq), C.byref(nn), C.byref(ff)) == 0
+        held = np.ctypeslib.as_array(pp, shape=(nn.value, 4))
+        kept = held.copy()
+        for _ in range(3):
+            s.Simulate()
+            s.readback_request()
+            s.readback_request()
+        s.synchronize()
+        assert ff.value == 4 and np.array_equal(held, kept)
+        assert s.readback_latest()[2] == 7
+    # what the viewer received is the simulation's state: the FP64 oracle after 4 frames
+    ref = oracle.simulate(sc, frames=4)
+    err = float(np.max(np.abs(pos[:, :3].astype(np.float64) - ref.x.reshape(-1, 3))))
+    say(f"  readback vs fp64 after 4 frames: {err:.3e}")
+    assert err <= 1e-3 * sc.extent()
 
 
 def _lists_equal(nb, rnb):

@@ -13,6 +13,7 @@ import sys
 import numpy as np
 import pytest
 
+import oracle
 from velvet_amd import scenes as S
 from velvet_amd.server import DONE, FAILED, QUEUED, REJECTED, RUNNING, Server
 from velvet_amd.solver import ClothSolver
@@ -148,6 +149,12 @@ def test_preempt_resume_equals_alone():
         for (sc, f), j in zip(js, ids):
             assert sv.status(j) == (DONE, f)
             assert np.array_equal(sv.result(j), alone(sc, f))
+        # the preempted job's result is the simulation's: the FP64 oracle over its horizon
+        sc, f = js[1]
+        ref = oracle.simulate(sc, frames=f).x.reshape(-1, 3)
+        err = float(np.max(np.abs(sv.result(ids[1]).astype(np.float64) - ref)))
+        say(f"  preempted job vs fp64 after {f} frames: {err:.3e}")
+        assert err <= 1e-3 * sc.extent()
         with pytest.raises(Exception):
             sv.preempt(ids[0])          # done jobs cannot be preempted
 

@@ -403,3 +403,237 @@ def test_rigid_translation_invariance():
     s2 = oracle.simulate(sc2, frames=3)
     d = s2.x.reshape(-1, 3) - shift - s1.x.reshape(-1, 3)
     assert np.max(np.abs(d)) < 2e-6        # inputs re-rounded to binary32 after the shift
+
+
+# ------------------------------------------------------------------ particle contacts
+SENT = 0xFFFFFFFF
+EPS = float(F32(1e-6))
+
+
+def _contacts(p, w, lists, x=None, diameter=0.1, friction=0.0, omega=1.0, K=8):
+    """One orc_collide_particles pass over hand-built neighbour lists (column-major, sentinel)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 3).copy()
+    n = p.shape[0]
+    x = p.copy() if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    nb = np.full((K, n), SENT, np.uint32)
+    for i, lst in enumerate(lists):
+        nb[: len(lst), i] = lst
+        if len(lst) < K:
+            nb[len(lst):, i] = [SENT] + [0] * (K - len(lst) - 1)   # garbage after the sentinel
+    nb = np.ascontiguousarray(nb)
+    oracle.lib().orc_collide_particles(n, oracle._p(p, oracle.f64), oracle._p(x, oracle.f64),
+                                       oracle._p(w, oracle.f64), oracle._p(nb, oracle.u32), K,
+                                       float(diameter), float(friction), float(omega))
+    return p
+
+
+def _pbd_pair_length(dist, d):
+    """Length after the textbook PBD projection of a pair to distance d, with the reference's eps
+    in the gradient denominator: d + (dist - d) * eps / (dist + eps)."""
+    return d + (dist - d) * EPS / (dist + EPS)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_contact_pair_separates_to_diameter(seed):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(3)
+    u /= np.linalg.norm(u)
+    d, dist = 0.1, 0.05
+    c = rng.standard_normal(3)
+    p = np.stack([c - 0.5 * dist * u, c + 0.5 * dist * u])
+    out = _contacts(p, [1.0, 1.0], [[1], [0]], diameter=d)
+    L = np.linalg.norm(out[1] - out[0])
+    assert L == pytest.approx(_pbd_pair_length(dist, d), abs=1e-14)
+    np.testing.assert_allclose(out.mean(0), c, atol=1e-15)             # equal masses: centre fixed
+    np.testing.assert_allclose(np.cross(out[1] - out[0], u), 0, atol=1e-15)   # along the axis
+
+
+def test_contact_zero_weight_partner_takes_nothing():
+    d, dist = 0.1, 0.03
+    p = np.array([[0.0, 0.0, 0.0], [dist, 0.0, 0.0]])
+    out = _contacts(p, [0.0, 1.0], [[1], [0]], diameter=d)
+    assert np.array_equal(out[0], p[0])                    # the pinned particle does not move
+    assert out[1, 0] == pytest.approx(_pbd_pair_length(dist, d), abs=1e-14)   # the free one takes all
+    # both pinned: the pair is skipped
+    out = _contacts(p, [0.0, 0.0], [[1], [0]], diameter=d)
+    assert np.array_equal(out, p)
+
+
+def test_contact_out_of_reach_neighbour_is_not_counted():
+    # a listed neighbour at or beyond the diameter neither moves the particle nor dilutes the
+    # Jacobi average of the contacts that do act
+    d = 0.1
+    p = np.array([[0.0, 0.0, 0.0], [0.06, 0.0, 0.0], [0.0, -0.1, 0.0], [0.0, 0.0, 0.15]])
+    w = [1.0, 0.0, 0.0, 0.0]
+    one = _contacts(p, w, [[1], [], [], []], diameter=d)
+    more = _contacts(p, w, [[1, 2, 3], [], [], []], diameter=d)
+    assert np.array_equal(one, more)
+    assert one[0, 0] == pytest.approx(0.06 - _pbd_pair_length(0.06, d), abs=1e-14)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.5])
+def test_contact_jacobi_average_of_pair_corrections(omega):
+    # two pinned partners: the free particle moves by omega x the mean of the two single-contact
+    # displacements (Jacobi average), each pinned above
+    d = 0.1
+    p = np.array([[0.0, 0.0, 0.0], [0.07, 0.0, 0.0], [0.0, 0.02, -0.05]])
+    w = [1.0, 0.0, 0.0]
+    a = _contacts(p, w, [[1], [], []], diameter=d)[0] - p[0]
+    b = _contacts(p, w, [[2], [], []], diameter=d)[0] - p[0]
+    both = _contacts(p, w, [[1, 2], [], []], diameter=d, omega=omega)[0] - p[0]
+    np.testing.assert_allclose(both, omega * 0.5 * (a + b), atol=1e-16)
+    # symmetric partners cancel
+    q = np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0], [-0.05, 0.0, 0.0]])
+    out = _contacts(q, w, [[1, 2], [], []], diameter=d)
+    np.testing.assert_allclose(out[0], 0.0, atol=1e-16)
+
+
+def test_contact_list_ends_at_sentinel():
+    d = 0.1
+    p = np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0], [0.0, 0.05, 0.0]])
+    w = [1.0, 0.0, 0.0]
+    n = 3
+    K = 4
+    nb = np.full((K, n), SENT, np.uint32)
+    nb[0, 0] = 1
+    nb[1, 0] = SENT
+    nb[2, 0] = 2               # after the sentinel: not part of the list
+    q = p.copy()
+    x = p.copy()
+    ww = np.asarray(w, np.float64)
+    oracle.lib().orc_collide_particles(n, oracle._p(q, oracle.f64), oracle._p(x, oracle.f64),
+                                       oracle._p(ww, oracle.f64), oracle._p(nb, oracle.u32), K, d, 0.0, 1.0)
+    np.testing.assert_array_equal(q, _contacts(p, w, [[1], [], []], diameter=d))
+
+
+@pytest.mark.parametrize("a,mu", [(0.002, 0.5), (0.02, 0.5), (0.02, 0.0)])
+def test_contact_friction_reduces_relative_tangential_motion(a, mu):
+    # particle 0 pressed against pinned particle 1 along x and sliding by a along y since the
+    # start of the substep; friction takes min(mu |c|, a) off its tangential displacement, with
+    # |c| the normal correction of the pair
+    d, dist = 0.1, 0.06
+    p = np.array([[0.0, 0.0, 0.0], [dist, 0.0, 0.0]])
+    x = p - np.array([[0.0, a, 0.0], [0.0, 0.0, 0.0]])
+    out = _contacts(p, [1.0, 0.0], [[1], [0]], x=x, diameter=d, friction=mu)
+    c = _pbd_pair_length(dist, d) - dist     # normal correction of particle 0 (pair pinned above)
+    assert out[0, 0] == pytest.approx(-c, abs=1e-14)
+    assert out[0, 1] == pytest.approx(-min(float(F32(mu)) * c, a), abs=1e-14)
+    assert out[0, 2] == 0.0
+    # the same sliding on both particles is no relative motion: no friction
+    x2 = p - np.array([[0.0, a, 0.0], [0.0, a, 0.0]])
+    out = _contacts(p, [1.0, 0.0], [[1], [0]], x=x2, diameter=d, friction=mu)
+    assert out[0, 1] == 0.0
+
+
+# ------------------------------------------------------------------ cube corners and edges
+M03 = float(F32(0.03))
+K08 = 1.0 - float(F32(0.2))      # the SDF scales rounded-band corrections by 0.2f
+
+
+def _cube_scene(point, scale=1.0, rotation=(0.0, 0.0, 0.0), margin=0.06):
+    col = S.Collider(S.CUBE, (0.0, 0.0, 0.0), (scale, scale, scale), rotation=rotation)
+    return tiny_scene([point], colliders=[col], collision_margin=margin)
+
+
+@pytest.mark.parametrize("signs", [(1, 1, 1), (-1, 1, -1), (1, -1, -1)])
+def test_cube_rounded_corner(signs):
+    # In the 0.03-wide corner band of a unit cube the SDF treats the corner as a sphere of radius
+    # 0.03 centred 0.03 inside the (margin-grown) corner and moves a point inside that sphere 20%
+    # of the way out along the radius. One frame applies it twice (pre-stabilisation, then the
+    # substep's pass): the distance to the centre closes 1 - 0.8^2 of its gap to 0.03.
+    margin = float(F32(0.06))
+    half = 0.5 + margin
+    sg = np.asarray(signs, float)
+    centre = sg * (half - M03)
+    off = np.array([0.004, 0.007, 0.005])        # from the corner sphere's centre, |off| < 0.03
+    q = (centre + sg * off).astype(F32).astype(np.float64)
+    l0 = np.linalg.norm(q - centre)
+    s = oracle.simulate(_cube_scene(q), frames=1)
+    out = s.x.reshape(3)
+    r = out - centre
+    assert np.linalg.norm(r) == pytest.approx(M03 - (M03 - l0) * K08 ** 2, abs=1e-12)
+    np.testing.assert_allclose(np.cross(r, q - centre), 0.0, atol=1e-15)   # radial
+
+
+def test_cube_edge_band_moves_a_fifth_of_the_way():
+    # near an edge (two faces within 0.03) but not a corner, the nearest face pushes with
+    # scalar 0.2: the gap to that face shrinks by 0.8 per pass, the other coordinates stay
+    margin = float(F32(0.06))
+    half = 0.5 + margin
+    q = np.array([half - 0.01, half - 0.02, 0.1]).astype(F32).astype(np.float64)
+    out = oracle.simulate(_cube_scene(q), frames=1).x.reshape(3)
+    assert half - out[0] == pytest.approx((half - q[0]) * K08 ** 2, abs=1e-12)
+    assert out[1] == q[1] and out[2] == q[2]
+
+
+def test_cube_rotation_and_scale_equivariance():
+    # the same local point in a rotated, uniformly scaled cube moves by the rotated, scaled
+    # local displacement
+    margin = 0.06
+    s_ = 2.0
+    half = 0.5 + margin / s_
+    local = np.array([half - 0.02, half - 0.025, -(half - 0.018)])   # inside the corner sphere
+    rot = (30.0, 20.0, 10.0)
+    Mw = S.transform_matrix((0.0, 0.0, 0.0), rot, (s_, s_, s_))
+    world = (Mw[:3, :3] @ local).astype(F32).astype(np.float64)
+    out = oracle.simulate(_cube_scene(world, scale=s_, rotation=rot, margin=margin), frames=1).x.reshape(3)
+    local_in = np.linalg.solve(Mw[:3, :3], world)
+    # reference: the unrotated unit-scale problem with margin / s, scaled back by s
+    ref = oracle.simulate(_cube_scene(local_in, margin=margin / s_), frames=1).x.reshape(3)
+    np.testing.assert_allclose(out, Mw[:3, :3] @ ref, atol=5e-7)
+    assert np.linalg.norm(out - world) > 1e-4          # it did move
+
+
+# ------------------------------------------------------------------ moving colliders
+def _sdf_pass(points, prev, collider, dt, margin=0.06, friction=0.1, col_dt=None):
+    """One orc_collide_sdf pass (pass time step dt; the collider's motion spans col_dt)."""
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3).copy()
+    x = np.ascontiguousarray(prev, np.float64).reshape(-1, 3)
+    cols = oracle.make_colliders([collider], frame_dt=col_dt)
+    oracle.lib().orc_collide_sdf(p.shape[0], oracle._p(p, oracle.f64), oracle._p(x, oracle.f64), cols, 1,
+                                 float(F32(margin)), float(F32(friction)), float(dt))
+    return p
+
+
+@pytest.mark.parametrize("mu", [0.05, 5.0])
+def test_translating_plane_drags_a_resting_particle(mu):
+    # a plane sliding along +x at speed u: its surface moves u h during a pass of length h (not
+    # u x the collider's frame time); a particle at rest inside it is dragged by min(mu c, u h)
+    T = float(S.FRAME_DT)
+    h = T / 4
+    u = 0.3
+    col = S.Collider(S.PLANE, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), last_position=(-u * T, 0.0, 0.0))
+    q = np.array([[0.3, -0.02, 0.1]])
+    out = _sdf_pass(q, q, col, h, friction=mu, col_dt=T)[0]
+    m = float(F32(0.06))
+    c = m + 0.02
+    mu32 = float(F32(mu))
+    assert out[1] == pytest.approx(m, abs=1e-15)
+    assert out[0] == pytest.approx(0.3 + min(mu32 * c, u * h), abs=1e-9)
+    assert out[2] == pytest.approx(0.1, abs=1e-12)
+
+
+def test_rotating_collider_drags_along_its_material_point():
+    # A collider rotating about y and translating: the material point now at p was at
+    # last * inverse(cur) * p one collider frame ago. With enough friction a resting particle in
+    # contact is carried by that motion scaled to the pass time (tangential to the plane's normal).
+    T = float(S.FRAME_DT)
+    h = T / 2
+    col = S.Collider(S.PLANE, (0.05, 0.0, -0.02), (1.0, 1.0, 1.0), rotation=(0.0, 12.0, 0.0),
+                     last_position=(0.0, 0.0, 0.0), last_rotation=(0.0, 9.0, 0.0))
+    q = np.array([[0.4, -0.01, 0.25]])
+    out = _sdf_pass(q, q, col, h, friction=50.0, col_dt=T)[0]
+    cur = col.cur_matrix().astype(F32).astype(np.float64)
+    last = col.last_matrix().astype(F32).astype(np.float64)
+    m = float(F32(0.06))
+    pc = np.array([q[0, 0], m, q[0, 2], 1.0])           # after the normal correction
+    was = last @ np.linalg.inv(cur) @ pc
+    move = (pc[:3] - was[:3]) * (h / T)
+    assert abs(move[1]) < 1e-12                          # y-rotation + xz translation: tangential
+    np.testing.assert_allclose(out, pc[:3] + move, atol=2e-7)
+    # the direction is the rigid motion's: about omega x r + translation for a small step
+    omega = np.radians(3.0) / T * np.array([0.0, 1.0, 0.0])
+    approx = (np.cross(omega, pc[:3] - np.array([0.05, 0.0, -0.02]))
+              + np.array([0.05, 0.0, -0.02]) / T) * h
+    np.testing.assert_allclose(move, approx, atol=0.05 * np.linalg.norm(approx))   # O(angle) terms

@@ -432,7 +432,7 @@ void Engine::restore(int32_t s, const void* buf, size_t bytes) {
 }
 
 Readback::~Readback() {
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < kBuffers; k++) {
         if (ev[k]) {
             (void)hipEventSynchronize(ev[k]);
             (void)hipEventDestroy(ev[k]);
@@ -450,28 +450,32 @@ void Engine::readback_request(int32_t s) {
     if (!n) return;
     if (!sg.rb || sg.rb->n != n) {
         sg.rb.reset(new Readback);
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < Readback::kBuffers; k++) {
             VTX_HIP(hipHostMalloc((void**)&sg.rb->host[k], sizeof(float4) * 2 * (size_t)n, hipHostMallocDefault));
             VTX_HIP(hipEventCreateWithFlags(&sg.rb->ev[k], hipEventDisableTiming));
         }
         sg.rb->n = n;
     }
     Readback& rb = *sg.rb;
-    const int k = rb.next;
+    int k = -1;
+    for (int t = 0; t < Readback::kBuffers; t++)
+        if (t != rb.handed && (k < 0 || rb.seq[t] < rb.seq[k])) k = t;
     VTX_HIP(hipMemcpyAsync(rb.host[k], pl_.X + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost, stream_));
     VTX_HIP(hipMemcpyAsync(rb.host[k] + n, pl_.N + sg.rg.p, sizeof(float4) * n, hipMemcpyDeviceToHost, stream_));
     VTX_HIP(hipEventRecord(rb.ev[k], stream_));
     rb.frame[k] = frames_;
     rb.pending[k] = true;
-    rb.next = k ^ 1;
+    rb.seq[k] = ++rb.nseq;
 }
 
 bool Engine::readback_latest(int32_t s, const float4** pos, const float4** nrm, uint32_t* n, uint64_t* frame) {
     Segment& sg = seg(s);
     if (!sg.rb) return false;
     Readback& rb = *sg.rb;
-    for (int t = 0; t < 2; t++) {
-        const int k = rb.next ^ 1 ^ t;           // newest request first
+    int order[Readback::kBuffers];
+    for (int t = 0; t < Readback::kBuffers; t++) order[t] = t;
+    std::sort(order, order + Readback::kBuffers, [&](int a, int b) { return rb.seq[a] > rb.seq[b]; });
+    for (int k : order) {                        // newest request first
         if (!rb.pending[k]) continue;
         const hipError_t e = hipEventQuery(rb.ev[k]);
         if (e == hipSuccess) {
@@ -479,6 +483,7 @@ bool Engine::readback_latest(int32_t s, const float4** pos, const float4** nrm, 
             *nrm = rb.host[k] + rb.n;
             *n = rb.n;
             *frame = rb.frame[k];
+            rb.handed = k;
             return true;
         }
         if (e != hipErrorNotReady) VTX_HIP(e);
@@ -1448,15 +1453,15 @@ int32_t Engine::stencil_segments() {
 }
 
 // ---------------------------------------------------------------------------- frames
-void Engine::stage_begin(const char* label, hipStream_t s) {
+void Engine::stage_begin(const char* kernel, const char* label, hipStream_t s) {
     if (!timing_) return;
     if (stage_used_ == stages_.size()) {
         StageTime st;
-        st.label = label;
         VTX_HIP(hipEventCreate(&st.a));
         VTX_HIP(hipEventCreate(&st.b));
         stages_.push_back(st);
     }
+    stages_[stage_used_].kernel = kernel;
     stages_[stage_used_].label = label;
     VTX_HIP(hipEventRecord(stages_[stage_used_].a, s));
 }
@@ -1471,27 +1476,27 @@ void Engine::stage_end(hipStream_t s) {
 void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
     // small segments: the whole build in one workgroup each, into the buffer the device-wide
     // sort of the other segments ends in
-    stage_begin("k_hash_small", s);
+    stage_begin("k_hash_small", "Solver_HashParticle+Solver_HashSort+Solver_HashBuildCell", s);
     VTX_HIP(launch_hash_small(d, in, passes_ & 1, s));
     stage_end(s);
     int src = passes_ & 1;
     if (d.nstiles) {   // segments on the device-wide path (every segment has sort tiles but those above)
-        stage_begin("k_hash_keys", s);
+        stage_begin("k_hash_keys", "Solver_HashParticle", s);
         VTX_HIP(launch_hash_keys(d, in, s));
         stage_end(s);
-        stage_begin("hash_sort", s);
+        stage_begin("k_sort_hist+k_scan_reduce+k_scan_apply+k_sort_scatter", "Solver_HashSort", s);
         src = 0;
         for (int pass = 0; pass < passes_; pass++) {
             VTX_HIP(launch_sort_pass(d, src, 8 * pass, s));
             src ^= 1;
         }
         stage_end(s);
-        stage_begin("hash_cells", s);
+        stage_begin("k_clear_cells+k_cell_bounds", "Solver_HashBuildCell", s);
         VTX_HIP(launch_clear_cells(d, s));
         VTX_HIP(launch_cell_bounds(d, in, src, s));
         stage_end(s);
     }
-    stage_begin("k_neighbors", s);
+    stage_begin("k_neighbors", "Solver_HashCache", s);
     VTX_HIP(launch_neighbors(d, in, tun_.nbr_order, tun_.gather_block, s));
     stage_end(s);
     sorted_src_ = src;
@@ -1501,7 +1506,7 @@ void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
 // In timing mode every launch group is bracketed by an event pair labelled with its kernel.
 void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s, hipEvent_t after_hash) {
     const float h = frame_dt / (float)substeps_;
-    stage_begin("k_frame_begin", s);
+    stage_begin("k_frame_begin", "Solver_CollideSDFs+Solver_Predict", s);
     VTX_HIP(launch_frame_begin(d, frame_dt, h, s));
     stage_end(s);
     if (after_hash && !any_selfcol_) VTX_HIP(hipEventRecord(after_hash, s));
@@ -1512,7 +1517,7 @@ void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s, hipE
             if (after_hash && sub == 0) VTX_HIP(hipEventRecord(after_hash, s));
         }
         if (need_collide_) {
-            stage_begin("k_collide", s);
+            stage_begin("k_collide", "Solver_CollideParticles+Solver_ApplyDeltas+Solver_CollideSDFs", s);
             VTX_HIP(launch_collide(d, cur, h, tun_.gather_block, s));
             stage_end(s);
             cur ^= 1;
@@ -1520,20 +1525,28 @@ void Engine::record_group(const DevState& d, float frame_dt, hipStream_t s, hipE
         const bool last_sub = sub == substeps_ - 1;
         const int epi_last = last_sub ? EPILOGUE_FINALIZE : EPILOGUE_FINALIZE_PREDICT;
         if (iterations_ == 0) {
-            stage_begin("k_iterate", s);
+            stage_begin("k_iterate", last_sub ? "Solver_Finalize" : "Solver_Finalize+Solver_Predict", s);
             VTX_HIP(launch_iterate(d, cur, h, epi_last, false, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
         for (int it = 0; it < iterations_; it++) {
             const int epi = it == iterations_ - 1 ? epi_last : EPILOGUE_NONE;
-            stage_begin("k_iterate", s);
+            stage_begin("k_iterate",
+                        epi == EPILOGUE_NONE
+                            ? "Solver_SolveStretch+Solver_SolveAttach+Solver_SolveBending+Solver_ApplyDeltas"
+                        : epi == EPILOGUE_FINALIZE
+                            ? "Solver_SolveStretch+Solver_SolveAttach+Solver_SolveBending+Solver_ApplyDeltas+"
+                              "Solver_Finalize"
+                            : "Solver_SolveStretch+Solver_SolveAttach+Solver_SolveBending+Solver_ApplyDeltas+"
+                              "Solver_Finalize+Solver_Predict",
+                        s);
             VTX_HIP(launch_iterate(d, cur, h, epi, true, tun_.lra_batch, s));
             stage_end(s);
             cur ^= 1;
         }
     }
-    stage_begin("k_normals", s);
+    stage_begin("k_normals", "Solver_UpdateNormals", s);
     VTX_HIP(launch_normals(d, s));
     stage_end(s);
 }
@@ -1591,6 +1604,7 @@ void Engine::simulate(float frame_dt) {
             VTX_HIP(hipEventElapsedTime(&ms, stages_[k].a, stages_[k].b));
             timing_ms_[stages_[k].label] += ms;
             timing_n_[stages_[k].label] += 1;
+            timing_kernel_[stages_[k].label] = stages_[k].kernel;
             timing_ms_["total"] += ms;
             timing_n_["total"] += 1;
         }
@@ -1810,6 +1824,7 @@ void Engine::set_timing(bool on) {
     timing_ = on;
     timing_ms_.clear();
     timing_n_.clear();
+    timing_kernel_.clear();
 }
 
 std::string Engine::timing_report() {
@@ -1819,7 +1834,10 @@ std::string Engine::timing_report() {
     o.precision(9);
     for (auto& kv : timing_ms_) {
         o << (first ? "" : ", ") << "\"" << kv.first << "\": {\"ms\": " << kv.second
-          << ", \"launches\": " << timing_n_[kv.first] << "}";
+          << ", \"launches\": " << timing_n_[kv.first];
+        auto k = timing_kernel_.find(kv.first);
+        if (k != timing_kernel_.end()) o << ", \"kernel\": \"" << k->second << "\"";
+        o << "}";
         first = false;
     }
     o << "}";

@@ -73,15 +73,20 @@ struct Ranges {
 
 // One independent simulation: the state of one reference VtClothSolverGPU
 // (Velvet/VtClothSolverGPU.hpp:207-231).
-// Double-buffered device -> pinned-host copies of a segment's positions and normals for a
-// viewer: each request fills the older buffer after the work queued so far.
+// Device -> pinned-host copies of a segment's positions and normals for a viewer, in three
+// buffers: each request fills the least recently requested buffer other than the one last
+// handed to the viewer, after the work queued so far, so the newest request and the buffer the
+// viewer holds are never overwritten.
 struct Readback {
-    float4* host[2] = {nullptr, nullptr};    // n positions (w = inverse mass), then n normals
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    uint64_t frame[2] = {0, 0};
-    bool pending[2] = {false, false};
+    static constexpr int kBuffers = 3;
+    float4* host[kBuffers] = {};             // n positions (w = inverse mass), then n normals
+    hipEvent_t ev[kBuffers] = {};
+    uint64_t frame[kBuffers] = {};
+    uint64_t seq[kBuffers] = {};             // request order (0 = never requested)
+    bool pending[kBuffers] = {};
     uint32_t n = 0;
-    int next = 0;
+    uint64_t nseq = 0;
+    int handed = -1;                         // buffer returned by the last readback_latest
     Readback() = default;
     Readback(const Readback&) = delete;
     Readback& operator=(const Readback&) = delete;
@@ -129,8 +134,12 @@ struct Tuning {
     bool hash_lds = true;     // small self-colliding segments build their hash in one workgroup
 };
 
+// A timed launch group: its kernel and the reference's timer labels of the stages it runs
+// (Velvet/Timer.hpp:242-260 ScopedTimerGPU labels, Velvet/GUI.cpp:32-51), '+'-joined where one
+// kernel fuses several of them.
 struct StageTime {
-    std::string label;
+    const char* kernel;
+    const char* label;
     hipEvent_t a, b;
 };
 
@@ -240,7 +249,7 @@ private:
     void record_frame(float frame_dt, hipStream_t s, bool lazy);
     void hash_build(const DevState& d, int in, hipStream_t s);
     void record_group(const DevState& d, float frame_dt, hipStream_t s, hipEvent_t after_hash = nullptr);
-    void stage_begin(const char* label, hipStream_t s);
+    void stage_begin(const char* kernel, const char* label, hipStream_t s);
     void stage_end(hipStream_t s);
     void check_seg(int32_t s) const;
     void free_tables();
@@ -303,6 +312,7 @@ private:
     size_t stage_used_ = 0;
     std::map<std::string, double> timing_ms_;
     std::map<std::string, long> timing_n_;
+    std::map<std::string, std::string> timing_kernel_;
 };
 
 }  // namespace vtx

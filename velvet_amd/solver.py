@@ -127,9 +127,19 @@ class ClothSolver:
 
     # -- registration, reference names (Velvet/VtClothSolverGPU.hpp:114-205)
     def AddCloth(self, xyz, tris, diameter, model=None, seg: int = 0) -> int:
+        """Register a cloth (Velvet/VtClothSolverGPU.hpp:114-156); returns its first particle.
+
+        ``model`` transforms the positions (InitializePositions, Velvet/VtClothSolverGPU.cu:30-40):
+        a 4x4 array is a matrix in the usual math layout (translation in ``model[:3, 3]``) and is
+        passed to the library column-major; a flat 16-vector is taken as already column-major
+        (glm's memory layout). None is the identity.
+        """
         x = np.ascontiguousarray(xyz, dtype=F32).reshape(-1)
         t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
-        m = None if model is None else np.ascontiguousarray(model, dtype=F32).reshape(16)
+        m = None
+        if model is not None:
+            ma = np.asarray(model, dtype=F32)
+            m = np.ascontiguousarray(ma.T if ma.shape == (4, 4) else ma, dtype=F32).reshape(16)
         off = C.c_int32()
         check(lib().vtx_add_cloth(self.h, seg, _fp(x), x.size // 3, _up(t), t.size // 3,
                                   _fp(m) if m is not None else None, float(diameter), C.byref(off)))
