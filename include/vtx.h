@@ -78,7 +78,7 @@ typedef struct vtx_solver vtx_solver;
 
 /* Layout version of the structs above and below; vtx_abi_version() returns the library's, so a
  * binding can check at load time that it was written against the same layouts. */
-#define VTX_ABI_VERSION 2
+#define VTX_ABI_VERSION 3
 
 void vtx_params_default(vtx_params* p);
 const char* vtx_last_error(void);
@@ -247,6 +247,33 @@ int vtx_run_hash(vtx_solver* s);
 int vtx_hash_table_size(vtx_solver* s, int32_t seg, uint32_t* M);
 int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* cell_end,
                   uint32_t* sorted_hash, uint32_t* sorted_index, uint32_t* nbr);
+
+/* Kernel-launch tier of the spatial hash. Replaces HashParams and HashObjects
+ * (Velvet/SpatialHashGPU.cuh:7-25, Velvet/SpatialHashGPU.cu:159-196), the free function the
+ * reference's SpatialHashGPU::Hash calls on raw device pointers. Every array pointer is device
+ * memory: positions / original_positions hold num_objects packed xyz floats (glm::vec3); on
+ * return particle_hash / particle_index hold the hashes sorted stably and the object indices in
+ * that order (the reference sorts them in place), cell_start / cell_end (table_size entries
+ * each) the first / one-past-last sorted position of every non-empty cell and 0xffffffff in
+ * both for an empty one, and neighbors (max_num_neighbors x num_objects, entry k of object i at
+ * neighbors[k * num_objects + i]) each object's list, ended by 0xffffffff unless full -- the
+ * same tables as vtx_read_hash. Accepted neighbours j != i have |p_i - p_j|^2 < cell_spacing2
+ * and |x0_i - x0_j|^2 > particle_diameter2. Limits: 1 <= max_num_neighbors <= 64,
+ * 1 <= table_size < 2^28, num_objects < 2^25. Asynchronous on hip_stream (a hipStream_t; NULL =
+ * the null stream); the input arrays must stay unchanged until the work is done.
+ * vtx_hash_objects takes its sort and walk workspace with hipMallocAsync on the stream and
+ * frees it there (hipFreeAsync), as the reference resizes its static sort storage. */
+typedef struct vtx_hash_params {
+    uint32_t num_objects;
+    uint32_t max_num_neighbors;
+    float cell_spacing;
+    float cell_spacing2;
+    int32_t table_size;
+    float particle_diameter2;
+} vtx_hash_params;
+int vtx_hash_objects(uint32_t* particle_hash, uint32_t* particle_index, uint32_t* cell_start, uint32_t* cell_end,
+                     uint32_t* neighbors, const float* positions, const float* original_positions,
+                     const vtx_hash_params* params, void* hip_stream);
 
 /* Per-stage GPU time under the reference's ScopedTimerGPU labels (Velvet/Timer.hpp:242-260,
  * Velvet/GUI.cpp:32-51). With timing on, frames are launched without the graph and every launch

@@ -35,3 +35,20 @@ def test_abi_version_matches_header():
 def test_struct_sizes_match_header():
     assert C.sizeof(_lib.VtxParams) == 4 * 20
     assert C.sizeof(_lib.VtxCollider) == 4 * (1 + 3 + 3 + 16 + 16 + 1)
+
+
+def test_hash_objects_rejects_bad_params_before_any_device_work():
+    # validation happens on the host: these calls return errors without touching the (fake)
+    # pointers, so they run without a GPU
+    L = _lib.lib()
+    fake = C.c_void_p(16)
+    base = dict(num_objects=10, max_num_neighbors=64, cell_spacing=0.1, cell_spacing2=0.01, table_size=20,
+                particle_diameter2=0.0025)
+    for bad, code in ((dict(max_num_neighbors=0), -1), (dict(max_num_neighbors=65), -1), (dict(table_size=0), -1),
+                      (dict(table_size=1 << 28), -1), (dict(num_objects=1 << 25), -5), (dict(cell_spacing=0.0), -1)):
+        prm = _lib.VtxHashParams(**{**base, **bad})
+        rc = L.vtx_hash_objects(fake, fake, fake, fake, fake, fake, fake, C.byref(prm), None)
+        assert rc == code, (bad, rc, L.vtx_last_error())
+    prm = _lib.VtxHashParams(**base)
+    assert L.vtx_hash_objects(fake, fake, None, fake, fake, fake, fake, C.byref(prm), None) == -1
+    assert L.vtx_hash_objects(fake, fake, fake, fake, fake, fake, fake, None, None) == -1

@@ -17,7 +17,7 @@ i32, u32, f32 = C.c_int32, C.c_uint32, C.c_float
 P = C.POINTER
 
 
-ABI_VERSION = 2   # VTX_ABI_VERSION of include/vtx.h for the structures below
+ABI_VERSION = 3   # VTX_ABI_VERSION of include/vtx.h for the structures below
 
 
 class VtxParams(C.Structure):
@@ -45,6 +45,11 @@ class VtxTuning(C.Structure):
     _fields_ = [("use_graph", i32), ("neighbor_order", i32), ("lra_batch", i32), ("xcd_remap", i32),
                 ("frame_ms", f32), ("gather_block", i32), ("streams", i32), ("lra_split", i32),
                 ("hash_lds", i32)]
+
+
+class VtxHashParams(C.Structure):
+    _fields_ = [("num_objects", u32), ("max_num_neighbors", u32), ("cell_spacing", f32), ("cell_spacing2", f32),
+                ("table_size", i32), ("particle_diameter2", f32)]
 
 
 class VtxServerStats(C.Structure):
@@ -106,6 +111,7 @@ _SIGS = {
     "vtx_run_hash": (i32, [C.c_void_p]),
     "vtx_hash_table_size": (i32, [C.c_void_p, i32, P(u32)]),
     "vtx_read_hash": (i32, [C.c_void_p, i32, P(u32), P(u32), P(u32), P(u32), P(u32)]),
+    "vtx_hash_objects": (i32, [C.c_void_p] * 7 + [P(VtxHashParams), C.c_void_p]),
     "vtx_set_timing": (i32, [C.c_void_p, i32]),
     "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "vtx_server_create": (i32, [P(VtxServerConfig), P(C.c_void_p)]),

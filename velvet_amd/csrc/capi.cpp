@@ -6,6 +6,7 @@
 
 #include "../../include/vtx.h"
 #include "engine.hpp"
+#include "launch.hpp"
 #include "server.hpp"
 
 struct vtx_solver {
@@ -391,6 +392,52 @@ int vtx_read_hash(vtx_solver* s, int32_t seg, uint32_t* cell_start, uint32_t* ce
                   uint32_t* sorted_index, uint32_t* nbr) {
     NEED(s);
     API_BEGIN s->engine.read_hash(seg, cell_start, cell_end, sorted_hash, sorted_index, nbr); API_END
+}
+
+// ---------------------------------------------------------------- kernel-launch tier
+static void hash_args(vtx::HashObjectsArgs& a, uint32_t* particle_hash, uint32_t* particle_index,
+                      uint32_t* cell_start, uint32_t* cell_end, uint32_t* neighbors, const float* positions,
+                      const float* original_positions, const vtx_hash_params* p) {
+    VTX_REQUIRE(p, VTX_EINVAL, "params is NULL");
+    VTX_REQUIRE(p->max_num_neighbors >= 1 && p->max_num_neighbors <= VTX_MAX_NEIGHBORS, VTX_EINVAL,
+                "max_num_neighbors must be in [1, 64]");
+    VTX_REQUIRE(p->table_size >= 1 && p->table_size < (1 << 28), VTX_EINVAL, "table_size must be in [1, 2^28)");
+    VTX_REQUIRE(p->num_objects < (1u << 25), VTX_ECAPACITY, "num_objects must be below 2^25");
+    VTX_REQUIRE(cell_start && cell_end, VTX_EINVAL, "cell_start / cell_end are NULL");
+    VTX_REQUIRE(p->num_objects == 0 || (particle_hash && particle_index && neighbors && positions && original_positions),
+                VTX_EINVAL, "an object array is NULL");
+    VTX_REQUIRE(p->cell_spacing > 0.f, VTX_EINVAL, "cell_spacing must be positive");
+    a.particle_hash = particle_hash;
+    a.particle_index = particle_index;
+    a.cell_start = cell_start;
+    a.cell_end = cell_end;
+    a.neighbors = neighbors;
+    a.positions = positions;
+    a.original_positions = original_positions;
+    a.num_objects = p->num_objects;
+    a.max_num_neighbors = p->max_num_neighbors;
+    a.table_size = (uint32_t)p->table_size;
+    a.cell_spacing = p->cell_spacing;
+    a.cell_spacing2 = p->cell_spacing2;
+    a.particle_diameter2 = p->particle_diameter2;
+}
+
+int vtx_hash_objects(uint32_t* particle_hash, uint32_t* particle_index, uint32_t* cell_start, uint32_t* cell_end,
+                     uint32_t* neighbors, const float* positions, const float* original_positions,
+                     const vtx_hash_params* params, void* hip_stream) {
+    API_BEGIN
+    vtx::HashObjectsArgs a{};
+    hash_args(a, particle_hash, particle_index, cell_start, cell_end, neighbors, positions, original_positions, params);
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    void* ws = nullptr;
+    const size_t bytes = vtx::hash_workspace_bytes(a.num_objects, a.table_size);
+    const hipError_t e = hipMallocAsync(&ws, bytes, st);
+    VTX_REQUIRE(e == hipSuccess, VTX_ENOMEM, "hipMallocAsync of the hash workspace failed");
+    const hipError_t r = vtx::hash_objects(a, ws, st);
+    const hipError_t f = hipFreeAsync(ws, st);
+    VTX_HIP(r);
+    VTX_HIP(f);
+    API_END
 }
 
 int vtx_set_timing(vtx_solver* s, int32_t on) {

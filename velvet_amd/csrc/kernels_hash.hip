@@ -55,19 +55,25 @@ __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
                                int_coord(p.z, sg.cell), sg.M, seg_mod(sg));
 }
 
-// Per-tile digit histogram into hist[hbase + digit * ntiles + tidx].
-__global__ __launch_bounds__(VTX_TILE) void k_sort_hist(DevState d, int src, int shift) {
+// Digit histogram of one sort tile (count keys from `keys`): the count of digit g goes to
+// out[g * stride].
+__device__ __forceinline__ void tile_hist(const uint32_t* __restrict__ keys, uint32_t count, int shift,
+                                          uint32_t* __restrict__ out, uint32_t stride) {
     __shared__ uint32_t cnt[256];
-    const SortTileDev t = d.stiles[blockIdx.x];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t* __restrict__ keys = d.keys[src] + t.begin;
     for (uint32_t r = 0; r < VTX_SORT_ITEMS; r++) {
         const uint32_t i = r * VTX_TILE + threadIdx.x;
-        if (i < t.count) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+        if (i < count) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    d.hist[t.hbase + threadIdx.x * t.ntiles + t.tidx] = cnt[threadIdx.x];
+    out[threadIdx.x * stride] = cnt[threadIdx.x];
+}
+
+// Per-tile digit histogram into hist[hbase + digit * ntiles + tidx].
+__global__ __launch_bounds__(VTX_TILE) void k_sort_hist(DevState d, int src, int shift) {
+    const SortTileDev t = d.stiles[blockIdx.x];
+    tile_hist(d.keys[src] + t.begin, t.count, shift, d.hist + t.hbase + t.tidx, t.ntiles);
 }
 
 // Wave-level exclusive scan helpers (wave64).
@@ -137,22 +143,21 @@ __global__ __launch_bounds__(VTX_TILE) void k_scan_apply(uint32_t* __restrict__ 
     }
 }
 
-// Stable scatter of one sort tile: rounds of 256 keys in order; inside a round, ranks come
-// from a wave match on the 8 digit bits (ballots) and the waves' per-digit counts.
-__global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, int shift) {
+// Stable scatter of one sort tile (count keys and values from kin / vin): rounds of 256 keys in
+// order; inside a round, ranks come from a wave match on the 8 digit bits (ballots) and the
+// waves' per-digit counts. Digit g of the tile starts at output position scanned[g * stride] + bias.
+__device__ __forceinline__ void tile_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                             uint32_t count, int shift, const uint32_t* __restrict__ scanned,
+                                             uint32_t stride, uint32_t bias, uint32_t* __restrict__ kout,
+                                             uint32_t* __restrict__ vout) {
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcnt[VTX_TILE / 64][256];
-    const SortTileDev t = d.stiles[blockIdx.x];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    base[tid] = d.hist[t.hbase + tid * t.ntiles + t.tidx];
-    const uint32_t* __restrict__ kin = d.keys[src] + t.begin;
-    const uint32_t* __restrict__ vin = d.vals[src] + t.begin;
-    uint32_t* __restrict__ kout = d.keys[src ^ 1];
-    uint32_t* __restrict__ vout = d.vals[src ^ 1];
+    base[tid] = scanned[tid * stride];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (uint32_t r = 0; r < VTX_SORT_ITEMS; r++) {
         const uint32_t i = r * VTX_TILE + tid;
-        const bool valid = i < t.count;
+        const bool valid = i < count;
         const uint32_t key = valid ? kin[i] : 0u;
         const uint32_t val = valid ? vin[i] : 0u;
         const uint32_t dg = (key >> shift) & 255u;
@@ -181,12 +186,18 @@ __global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, 
         }
         __syncthreads();
         if (valid) {
-            const uint32_t dst = wcnt[wave][dg] + rank + t.bias;
+            const uint32_t dst = wcnt[wave][dg] + rank + bias;
             kout[dst] = key;
             vout[dst] = val;
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(VTX_TILE) void k_sort_scatter(DevState d, int src, int shift) {
+    const SortTileDev t = d.stiles[blockIdx.x];
+    tile_scatter(d.keys[src] + t.begin, d.vals[src] + t.begin, t.count, shift, d.hist + t.hbase + t.tidx, t.ntiles,
+                 t.bias, d.keys[src ^ 1], d.vals[src ^ 1]);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
@@ -406,6 +417,103 @@ __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (size_t)((i & 0x1FFFFFFu) << 5));
 }
 
+// The 27-cell walk of one particle (CacheNeighbors_Kernel, Velvet/SpatialHashGPU.cu:94-129): p / o
+// its predicted / initial position, id its index, sP the 32-B sorted records (position with
+// w = index at 2i, initial position at 2i + 1), cells the (start, end) table of M cells, nb its
+// column (entry k at nb[k * n]). crange is the workgroup's LDS cache of cell ranges, one
+// column per thread.
+template <int BS>
+__device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, const uint32_t id,
+                                              const float4* __restrict__ sP, const uint2* __restrict__ cells,
+                                              uint32_t* __restrict__ nb, const uint32_t n, const float cell,
+                                              const float cell2, const float diam2, const uint32_t M,
+                                              const uint64_t mc, const uint32_t K,
+                                              uint32_t (&crange)[VTX_NB_CACHE][BS]) {
+#pragma clang fp contract(off)
+    const float4* __restrict__ sX0 = sP + 1;
+    const int32_t ix = int_coord(p.x, cell), iy = int_coord(p.y, cell), iz = int_coord(p.z, cell);
+    uint32_t mask = 0, total = 0, nz = 0;
+    // VTX_NB_BATCH cell ranges in flight at a time, each one 8-byte load at a 32-bit byte offset
+    // (tables have M < 2^28 cells)
+#pragma unroll
+    for (int q0 = 0; q0 < 27; q0 += VTX_NB_BATCH) {
+        uint64_t cv[VTX_NB_BATCH];
+#pragma unroll
+        for (int r = 0; r < VTX_NB_BATCH; r++) {
+            const int q = q0 + r;
+            const uint32_t hq = hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, M, mc);
+            cv[r] = *reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(cells) + (hq << 3));
+        }
+#pragma unroll
+        for (int r = 0; r < VTX_NB_BATCH; r++) {
+            const int q = q0 + r;
+            const uint32_t cx = (uint32_t)cv[r], cy = (uint32_t)(cv[r] >> 32);
+            const uint32_t len = cx == VTX_INVALID ? 0u : min(cy, cx + K) - cx;
+            if (len && nz < VTX_NB_CACHE) crange[nz][threadIdx.x] = cx | len << 25;
+            nz += len ? 1u : 0u;
+            mask |= (len ? 1u : 0u) << q;
+            total += len;
+        }
+    }
+    if (total == 0) {
+        nb[0] = VTX_INVALID;
+        return;
+    }
+    // position of the next candidate (called only while candidates remain); moving on to the
+    // next non-empty cell is branch-free unless that cell lies beyond the LDS cache
+    uint32_t z = 0, s = 0, left = 0;
+    auto next = [&]() -> uint32_t {
+        const bool sw = left == 0;
+        const uint32_t e = crange[z < VTX_NB_CACHE ? z : VTX_NB_CACHE - 1u][threadIdx.x];
+        uint32_t st = e & 0x1FFFFFFu, ln = e >> 25;
+        if (sw && z >= VTX_NB_CACHE) {
+            const int q = __builtin_ctz(mask);
+            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, M, mc)];
+            st = c.x;
+            ln = min(c.y, c.x + K) - c.x;
+        }
+        mask = sw ? (mask & (mask - 1u)) : mask;
+        z += sw ? 1u : 0u;
+        s = sw ? st : s;
+        left = (sw ? ln : left) - 1u;
+        return s++;
+    };
+    // candidate test in walk order, one predicate and one predicated store; a full list takes
+    // no more entries
+    uint32_t k = 0;
+    auto take = [&](float4 qp, float4 r) {
+        const uint32_t j = __float_as_uint(qp.w);
+        const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
+        const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+        const bool acc = (j != id) & (d2 < cell2) & (e2 > diam2) & (k < K);
+        if (acc) nb[(size_t)k * n] = j;
+        k += acc ? 1u : 0u;
+    };
+    // two candidates per trip with alternating load buffers: the loads of the next candidate
+    // are in flight while the current one is tested
+    uint32_t cur = next();
+    float4 qa = ld_sorted(sP, cur), ra = ld_sorted(sX0, cur), qb, rb;
+    for (uint32_t it = 0; it < total; it += 2) {
+        const bool two = it + 1 < total;
+        if (two) {
+            cur = next();
+            qb = ld_sorted(sP, cur);
+            rb = ld_sorted(sX0, cur);
+        }
+        take(qa, ra);
+        if (!two) break;
+        if (it + 2 < total) {
+            cur = next();
+            qa = ld_sorted(sP, cur);
+            ra = ld_sorted(sX0, cur);
+        }
+        take(qb, rb);
+    }
+    if (k < K) nb[(size_t)k * n] = VTX_INVALID;                  // a full list has no sentinel
+}
+
 template <int ORDER, int BS>
 __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __restrict__ tiles,
                                                   const GridTileDev* __restrict__ gtiles, int in) {
@@ -437,89 +545,156 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const float4 o = sorted ? sX0[2 * i] : d.X0[sg.pbase + i];
     const uint32_t id = sorted ? __float_as_uint(p.w) : i;
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    const int32_t ix = int_coord(p.x, sg.cell), iy = int_coord(p.y, sg.cell), iz = int_coord(p.z, sg.cell);
-    const uint64_t mc = seg_mod(sg);
-    const uint32_t K = sg.max_nbr;
-    uint32_t mask = 0, total = 0, nz = 0;
-    // VTX_NB_BATCH cell ranges in flight at a time, each one 8-byte load at a 32-bit byte offset
-    // (tables have M < 2^28 cells)
-#pragma unroll
-    for (int q0 = 0; q0 < 27; q0 += VTX_NB_BATCH) {
-        uint64_t cv[VTX_NB_BATCH];
-#pragma unroll
-        for (int r = 0; r < VTX_NB_BATCH; r++) {
-            const int q = q0 + r;
-            const uint32_t hq = hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc);
-            cv[r] = *reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(cells) + (hq << 3));
-        }
-#pragma unroll
-        for (int r = 0; r < VTX_NB_BATCH; r++) {
-            const int q = q0 + r;
-            const uint32_t cx = (uint32_t)cv[r], cy = (uint32_t)(cv[r] >> 32);
-            const uint32_t len = cx == VTX_INVALID ? 0u : min(cy, cx + K) - cx;
-            if (len && nz < VTX_NB_CACHE) crange[nz][threadIdx.x] = cx | len << 25;
-            nz += len ? 1u : 0u;
-            mask |= (len ? 1u : 0u) << q;
-            total += len;
+    neighbor_walk<BS>(p, o, id, sP, cells, nb, n, sg.cell, sg.cell2, sg.diam2, sg.M, seg_mod(sg), sg.max_nbr, crange);
+}
+
+// ---------------------------------------------------------------- kernel-launch tier
+// HashObjects on raw device arrays (Velvet/SpatialHashGPU.cu:159-196, SpatialHashGPU.cuh:17-25):
+// one object set, positions and initial positions as packed xyz triples (glm::vec3), the
+// sort's key/value ping-pong partly in the caller's particle_hash / particle_index and partly in
+// a workspace, which also holds the interleaved cell table and the 32-B sorted records that
+// the walk reads (one cache line per candidate).
+
+__device__ __forceinline__ float4 ld_xyz(const float* __restrict__ a, uint32_t i, float w) {
+    return make_float4(a[3 * (size_t)i], a[3 * (size_t)i + 1], a[3 * (size_t)i + 2], w);
+}
+
+// ComputeParticleHash_Kernel (Velvet/SpatialHashGPU.cu:34-42).
+__global__ __launch_bounds__(VTX_TILE) void k_raw_keys(const float* __restrict__ pos, uint32_t n, float cell,
+                                                       uint32_t M, uint64_t mc, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * VTX_TILE + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = ld_xyz(pos, i, 0.f);
+    keys[i] = hash_coords(int_coord(p.x, cell), int_coord(p.y, cell), int_coord(p.z, cell), M, mc);
+    vals[i] = i;
+}
+
+// Sort tile b covers keys [b * VTX_SORT_TILE, ...); its digit g count sits at hist[g * ntiles + b].
+__global__ __launch_bounds__(VTX_TILE) void k_raw_sort_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                            uint32_t* __restrict__ hist) {
+    const uint32_t b = blockIdx.x, begin = b * VTX_SORT_TILE;
+    tile_hist(keys + begin, min(n - begin, (uint32_t)VTX_SORT_TILE), shift, hist + b, gridDim.x);
+}
+
+__global__ __launch_bounds__(VTX_TILE) void k_raw_sort_scatter(const uint32_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin, uint32_t n,
+                                                               int shift, const uint32_t* __restrict__ hist,
+                                                               uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    const uint32_t b = blockIdx.x, begin = b * VTX_SORT_TILE;
+    tile_scatter(kin + begin, vin + begin, min(n - begin, (uint32_t)VTX_SORT_TILE), shift, hist + b, gridDim.x, 0u,
+                 kout, vout);
+}
+
+// FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) into the interleaved table (cells were
+// filled with ~0u), plus the sorted records.
+__global__ __launch_bounds__(VTX_TILE) void k_raw_cell_bounds(const uint32_t* __restrict__ skeys,
+                                                              const uint32_t* __restrict__ svals, uint32_t n,
+                                                              const float* __restrict__ pos,
+                                                              const float* __restrict__ pos0,
+                                                              uint2* __restrict__ cells, float4* __restrict__ rec) {
+    const uint32_t g = blockIdx.x * VTX_TILE + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t key = skeys[g];
+    if (g == 0) {
+        cells[key].x = 0;
+    } else {
+        const uint32_t prev = skeys[g - 1];
+        if (key != prev) {
+            cells[key].x = g;
+            cells[prev].y = g;
         }
     }
-    if (total == 0) {
-        nb[0] = VTX_INVALID;
-        return;
+    if (g == n - 1) cells[key].y = n;
+    const uint32_t id = svals[g];
+    rec[2 * (size_t)g] = ld_xyz(pos, id, __uint_as_float(id));
+    rec[2 * (size_t)g + 1] = ld_xyz(pos0, id, 0.f);
+}
+
+// CacheNeighbors_Kernel (Velvet/SpatialHashGPU.cu:79-130), threads in sorted order.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_raw_neighbors(const float4* __restrict__ rec, const uint2* __restrict__ cells,
+                                                      uint32_t* __restrict__ nbr, uint32_t n, float cell, float cell2,
+                                                      float diam2, uint32_t M, uint64_t mc, uint32_t K) {
+    __shared__ uint32_t crange[VTX_NB_CACHE][BS];
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = rec[2 * (size_t)i], o = rec[2 * (size_t)i + 1];
+    const uint32_t id = __float_as_uint(p.w);
+    neighbor_walk<BS>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, crange);
+}
+
+// The interleaved table to the caller's cell_start / cell_end (empty cells: ~0u in both).
+__global__ __launch_bounds__(VTX_TILE) void k_raw_split_cells(const uint2* __restrict__ cells, uint32_t M,
+                                                              uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    for (uint32_t i = blockIdx.x * VTX_TILE + threadIdx.x; i < M; i += gridDim.x * VTX_TILE) {
+        const uint2 c = cells[i];
+        start[i] = c.x;
+        end[i] = c.y;
     }
-    // position of the next candidate (called only while candidates remain); moving on to the
-    // next non-empty cell is branch-free unless that cell lies beyond the LDS cache
-    uint32_t z = 0, s = 0, left = 0;
-    auto next = [&]() -> uint32_t {
-        const bool sw = left == 0;
-        const uint32_t e = crange[z < VTX_NB_CACHE ? z : VTX_NB_CACHE - 1u][threadIdx.x];
-        uint32_t st = e & 0x1FFFFFFu, ln = e >> 25;
-        if (sw && z >= VTX_NB_CACHE) {
-            const int q = __builtin_ctz(mask);
-            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, sg.M, mc)];
-            st = c.x;
-            ln = min(c.y, c.x + K) - c.x;
-        }
-        mask = sw ? (mask & (mask - 1u)) : mask;
-        z += sw ? 1u : 0u;
-        s = sw ? st : s;
-        left = (sw ? ln : left) - 1u;
-        return s++;
-    };
-    // candidate test in walk order, one predicate and one predicated store; a full list takes
-    // no more entries
-    uint32_t k = 0;
-    auto take = [&](float4 qp, float4 r) {
-        const uint32_t j = __float_as_uint(qp.w);
-        const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
-        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
-        const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-        const bool acc = (j != id) & (d2 < sg.cell2) & (e2 > sg.diam2) & (k < K);
-        if (acc) nb[(size_t)k * n] = j;
-        k += acc ? 1u : 0u;
-    };
-    // two candidates per trip with alternating load buffers: the loads of the next candidate
-    // are in flight while the current one is tested
-    uint32_t cur = next();
-    float4 qa = ld_sorted(sP, cur), ra = ld_sorted(sX0, cur), qb, rb;
-    for (uint32_t it = 0; it < total; it += 2) {
-        const bool two = it + 1 < total;
-        if (two) {
-            cur = next();
-            qb = ld_sorted(sP, cur);
-            rb = ld_sorted(sX0, cur);
-        }
-        take(qa, ra);
-        if (!two) break;
-        if (it + 2 < total) {
-            cur = next();
-            qa = ld_sorted(sP, cur);
-            ra = ld_sorted(sX0, cur);
-        }
-        take(qb, rb);
+}
+
+static uint64_t mod_constant(uint32_t M) {   // ceil(2^64 / M) mod 2^64 (M = 1 gives 0: h % 1 = 0)
+    return M > 1 ? ~0ull / M + 1ull : 0ull;
+}
+
+size_t hash_workspace_bytes(uint32_t n, uint32_t M) {
+    const size_t tiles = (n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
+    const size_t nhist = 256 * tiles, nparts = (nhist + VTX_SCAN_BLOCK - 1) / VTX_SCAN_BLOCK;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up(32 * (size_t)n) + up(8 * (size_t)M) + up(8 * (size_t)n) + up(4 * nhist) + up(4 * nparts);
+}
+
+hipError_t hash_objects(const HashObjectsArgs& a, void* workspace, hipStream_t s) {
+    const uint32_t n = a.num_objects, M = a.table_size;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char* w = static_cast<char*>(workspace);
+    float4* rec = reinterpret_cast<float4*>(w);
+    w += up(32 * (size_t)n);
+    uint2* cells = reinterpret_cast<uint2*>(w);
+    w += up(8 * (size_t)M);
+    uint32_t* kalt = reinterpret_cast<uint32_t*>(w);
+    uint32_t* valt = kalt + n;
+    w += up(8 * (size_t)n);
+    const uint32_t tiles = (n + VTX_SORT_TILE - 1) / VTX_SORT_TILE;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w);
+    w += up(4 * (size_t)256 * tiles);
+    uint32_t* part = reinterpret_cast<uint32_t*>(w);
+
+    // stable LSD radix sort on ceil(log2 M) bits (Velvet/SpatialHashGPU.cu:179), 8 per pass;
+    // the keys start on the side that makes the last pass end in the caller's arrays
+    const uint32_t bits = M > 1 ? 32u - (uint32_t)__builtin_clz(M - 1u) : 0u;
+    const int passes = (int)((bits + 7) / 8);
+    uint32_t* k[2] = {a.particle_hash, kalt};
+    uint32_t* v[2] = {a.particle_index, valt};
+    int src = passes & 1;
+    const uint32_t blocks = (n + VTX_TILE - 1) / VTX_TILE;
+    const uint64_t mc = mod_constant(M);
+    if (n) hipLaunchKernelGGL(k_raw_keys, dim3(blocks), dim3(VTX_TILE), 0, s, a.positions, n, a.cell_spacing, M, mc,
+                              k[src], v[src]);
+    for (int pass = 0; pass < passes && n; pass++) {
+        hipLaunchKernelGGL(k_raw_sort_hist, dim3(tiles), dim3(VTX_TILE), 0, s, (const uint32_t*)k[src], n, 8 * pass,
+                           hist);
+        hipError_t e = launch_scan(hist, 256 * tiles, part, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_raw_sort_scatter, dim3(tiles), dim3(VTX_TILE), 0, s, (const uint32_t*)k[src],
+                           (const uint32_t*)v[src], n, 8 * pass, (const uint32_t*)hist, k[src ^ 1], v[src ^ 1]);
+        src ^= 1;
     }
-    if (k < K) nb[(size_t)k * n] = VTX_INVALID;                  // a full list has no sentinel
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(cells), 2 * M, VTX_INVALID, s);
+    if (e != hipSuccess) return e;
+    if (n) {
+        hipLaunchKernelGGL(k_raw_cell_bounds, dim3(blocks), dim3(VTX_TILE), 0, s, (const uint32_t*)a.particle_hash,
+                           (const uint32_t*)a.particle_index, n, a.positions, a.original_positions, cells, rec);
+        hipLaunchKernelGGL((k_raw_neighbors<128>), dim3((n + 127) / 128), dim3(128), 0, s, (const float4*)rec,
+                           (const uint2*)cells, a.neighbors, n, a.cell_spacing, a.cell_spacing2, a.particle_diameter2,
+                           M, mc, a.max_num_neighbors);
+    }
+    uint32_t sb = (M + VTX_TILE - 1) / VTX_TILE;
+    if (sb > 4096) sb = 4096;
+    if (sb) hipLaunchKernelGGL(k_raw_split_cells, dim3(sb), dim3(VTX_TILE), 0, s, (const uint2*)cells, M, a.cell_start,
+                               a.cell_end);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- host launchers

@@ -28,4 +28,20 @@ hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s)
 enum { NBR_PARTICLE = 0, NBR_SORTED = 1, NBR_PATCH = 2 };
 hipError_t launch_neighbors(const DevState& d, int in, int order, int block, hipStream_t s);
 
+// HashObjects of the kernel-launch tier (Velvet/SpatialHashGPU.cuh:7-25) on raw device arrays.
+struct HashObjectsArgs {
+    uint32_t* particle_hash;         // n: sorted hashes on return
+    uint32_t* particle_index;        // n: object indices in sorted order on return
+    uint32_t* cell_start;            // M
+    uint32_t* cell_end;              // M
+    uint32_t* neighbors;             // max_num_neighbors x n, column-major
+    const float* positions;          // n xyz
+    const float* original_positions; // n xyz
+    uint32_t num_objects, max_num_neighbors, table_size;
+    float cell_spacing, cell_spacing2, particle_diameter2;
+};
+// Workspace bytes hash_objects needs for n objects and M cells (256-B aligned parts).
+size_t hash_workspace_bytes(uint32_t n, uint32_t M);
+hipError_t hash_objects(const HashObjectsArgs& a, void* workspace, hipStream_t s);
+
 }  // namespace vtx
