@@ -77,21 +77,35 @@ def _contact_pairs(sc, x, v, h):
     return pairs, len(inv)
 
 
-def _run_pair(sc, moving, vdown, frames, frame_dt=None, stencil_expected=None):
-    """GPU and FP64 runs of sc with the particles in `moving` given velocity (0, vdown, 0)."""
+def _start_velocities(sc, moving, vdown, amp=0.0, seed=5):
+    """Initial velocities: (0, vdown, 0) for the particles in `moving`, plus, with amp > 0, a
+    seeded tangential (x, z) velocity U(-amp, amp) x d / h on every particle, where h is the
+    substep of one 1/60 s frame at five substeps."""
+    v = np.zeros((sc.n, 3), F32)
+    if amp > 0:
+        d = float(sc.params.particle_diameter)
+        h = float(S.FRAME_DT) / 5
+        rng = np.random.default_rng(seed)
+        t = (rng.uniform(-1.0, 1.0, (sc.n, 3)) * amp * d / h).astype(F32)
+        v[:, 0], v[:, 2] = t[:, 0], t[:, 2]
+    v[moving, 1] = F32(vdown)
+    return v
+
+
+def _run_pair(sc, v0, frames, frame_dt=None, stencil_expected=None):
+    """GPU and FP64 runs of sc from initial velocities v0 (n x 3)."""
     with ClothSolver() as s:
         s.load_scene(sc)
         if stencil_expected is not None:
             assert s.stencil_segments() == stencil_expected
-        vel = np.array([0.0, vdown, 0.0], F32)
-        for i in moving:
-            s.set_particle(int(i), vel=vel)
+        for i in np.nonzero(np.any(v0 != 0, axis=1))[0]:
+            s.set_particle(int(i), vel=v0[i])
         for _ in range(frames):
             s.Simulate(frame_dt)
         s.synchronize()
         xg = s.positions().astype(np.float64)
     sim = oracle.OracleSim(sc, frame_dt)
-    sim.v.reshape(-1, 3)[moving, 1] = float(F32(vdown))
+    sim.v.reshape(-1, 3)[:] = v0.astype(np.float64)
     sim.run(frames)
     return xg, sim.x.reshape(-1, 3)
 
@@ -103,17 +117,26 @@ def test_contacts_one_substep_vs_fp64(kind):
     h = F32(S.FRAME_DT / F32(5))
     d = float(sc.params.particle_diameter)
     vdown = -0.07 * d / float(h)
+    # Every particle also gets a small tangential velocity. Started from rest shapes, the
+    # predicted positions of a uniformly moving region are a translation of the rest positions,
+    # so its stretch constraints sit exactly at rest in binary32 and are skipped and not
+    # counted (the reference's `distance != rest` test, cu:85), while in FP64 they are off rest
+    # by a rounding error and counted: the averaging counts then differ next to the fold, where
+    # other constraints carry large corrections (1.9e-4 in one substep, reproduced by an FP64
+    # run that tests equality in binary32). Tangential motion of 1e-2 d per substep keeps every
+    # length away from its rest value and changes the layer gap only at second order.
     x0 = S.flatten(sc)["x"].astype(np.float64)
-    v0 = np.zeros_like(x0)
-    v0[moving, 1] = float(F32(vdown))
-    pairs, parts = _contact_pairs(sc, x0, v0, float(h))
+    v0 = _start_velocities(sc, moving, vdown, amp=1e-2)
+    pairs, parts = _contact_pairs(sc, x0, v0.astype(np.float64), float(h))
     say(f"contacts {kind}: {pairs} contact pairs, {parts} of {sc.n} particles in contact at substep 0")
     assert parts >= 0.1 * sc.n and pairs >= 500
-    xg, xr = _run_pair(sc, moving, vdown, 1, frame_dt=h, stencil_expected=1 if kind == "folded" else 0)
+    xg, xr = _run_pair(sc, v0, 1, frame_dt=h, stencil_expected=1 if kind == "folded" else 0)
     err = float(np.max(np.abs(xg - xr)))
     say(f"  max err {err:.3e} (extent {sc.extent():.3f})")
     assert err <= 1e-5 * sc.extent()
-    assert float(np.max(np.abs(xr - x0 - v0 * float(h)))) > 1e-4   # the contacts moved the cloth
+    free = x0 + v0.astype(np.float64) * float(h) + np.asarray(sc.params.gravity) * float(h) ** 2
+    # the contacts moved the upper layer (no collider touches it): half of the 0.02 d overlap
+    assert float(np.max(np.abs(xr[moving] - free[moving]))) > 4e-3 * d
 
 
 @pytest.mark.parametrize("kind", ["stacked", "folded"])
@@ -122,7 +145,7 @@ def test_contacts_ten_frames_vs_fp64(kind):
     d = float(sc.params.particle_diameter)
     h = float(S.FRAME_DT) / sc.params.num_substeps
     vdown = -0.07 * d / h
-    xg, xr = _run_pair(sc, moving, vdown, 10)
+    xg, xr = _run_pair(sc, _start_velocities(sc, moving, vdown), 10)
     err = float(np.max(np.abs(xg - xr)))
     say(f"contacts {kind} 10 frames: max err {err:.3e} (extent {sc.extent():.3f})")
     assert np.all(np.isfinite(xg))
