@@ -275,6 +275,23 @@ int vtx_hash_objects(uint32_t* particle_hash, uint32_t* particle_index, uint32_t
                      uint32_t* neighbors, const float* positions, const float* original_positions,
                      const vtx_hash_params* params, void* hip_stream);
 
+/* vtx_hash_objects without any allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (work recorded under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, limits and results as
+ * vtx_hash_objects; the sort and walk workspace is a fixed device array of
+ * VTX_HASH_SCRATCH_LIMIT bytes, one per GPU, held from the first call on that GPU until the
+ * process ends. It needs 40.5 B per object plus 8 B per table cell (each part rounded up to
+ * 256 B): e.g. 1,048,576 objects with 2^21 cells fit; larger calls return VTX_ECAPACITY before
+ * any device work. The workspace is that of the current device (hipGetDevice), so hip_stream
+ * must belong to it. Calls on one GPU share its array: two calls whose work overlaps on that
+ * GPU (different streams, or host threads without ordering) race and give undefined tables;
+ * calls queued in order on one stream are safe. Nothing in the array needs clearing between
+ * calls. */
+#define VTX_HASH_SCRATCH_LIMIT (64u << 20)
+int vtx_hash_objects_noalloc(uint32_t* particle_hash, uint32_t* particle_index, uint32_t* cell_start,
+                             uint32_t* cell_end, uint32_t* neighbors, const float* positions,
+                             const float* original_positions, const vtx_hash_params* params, void* hip_stream);
+
 /* Per-stage GPU time under the reference's ScopedTimerGPU labels (Velvet/Timer.hpp:242-260,
  * Velvet/GUI.cpp:32-51). With timing on, frames are launched without the graph and every launch
  * group is bracketed by a HIP event pair on the solver's stream; vtx_read_timing returns JSON

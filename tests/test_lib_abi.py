@@ -52,3 +52,19 @@ def test_hash_objects_rejects_bad_params_before_any_device_work():
     prm = _lib.VtxHashParams(**base)
     assert L.vtx_hash_objects(fake, fake, None, fake, fake, fake, fake, C.byref(prm), None) == -1
     assert L.vtx_hash_objects(fake, fake, fake, fake, fake, fake, fake, None, None) == -1
+
+
+def test_hash_objects_noalloc_rejects_oversized_calls_before_any_device_work():
+    # the fixed workspace holds 40.5 B per object plus 8 B per cell (64 MiB): larger calls are
+    # refused on the host, so these run without a GPU
+    L = _lib.lib()
+    fake = C.c_void_p(16)
+    base = dict(num_objects=1 << 21, max_num_neighbors=64, cell_spacing=0.1, cell_spacing2=0.01,
+                table_size=1 << 22, particle_diameter2=0.0025)
+    prm = _lib.VtxHashParams(**base)
+    assert L.vtx_hash_objects_noalloc(fake, fake, fake, fake, fake, fake, fake, C.byref(prm), None) == -5
+    assert b"scratch" in L.vtx_last_error()
+    prm = _lib.VtxHashParams(**{**base, "num_objects": 1 << 20, "table_size": 1 << 24})
+    assert L.vtx_hash_objects_noalloc(fake, fake, fake, fake, fake, fake, fake, C.byref(prm), None) == -5
+    prm = _lib.VtxHashParams(**{**base, "max_num_neighbors": 0})
+    assert L.vtx_hash_objects_noalloc(fake, fake, fake, fake, fake, fake, fake, C.byref(prm), None) == -1

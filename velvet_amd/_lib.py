@@ -112,6 +112,7 @@ _SIGS = {
     "vtx_hash_table_size": (i32, [C.c_void_p, i32, P(u32)]),
     "vtx_read_hash": (i32, [C.c_void_p, i32, P(u32), P(u32), P(u32), P(u32), P(u32)]),
     "vtx_hash_objects": (i32, [C.c_void_p] * 7 + [P(VtxHashParams), C.c_void_p]),
+    "vtx_hash_objects_noalloc": (i32, [C.c_void_p] * 7 + [P(VtxHashParams), C.c_void_p]),
     "vtx_set_timing": (i32, [C.c_void_p, i32]),
     "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "vtx_server_create": (i32, [P(VtxServerConfig), P(C.c_void_p)]),

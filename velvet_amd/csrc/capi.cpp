@@ -440,6 +440,22 @@ int vtx_hash_objects(uint32_t* particle_hash, uint32_t* particle_index, uint32_t
     API_END
 }
 
+static_assert(VTX_HASH_SCRATCH_BYTES == VTX_HASH_SCRATCH_LIMIT, "scratch size in launch.hpp and vtx.h differ");
+
+int vtx_hash_objects_noalloc(uint32_t* particle_hash, uint32_t* particle_index, uint32_t* cell_start,
+                             uint32_t* cell_end, uint32_t* neighbors, const float* positions,
+                             const float* original_positions, const vtx_hash_params* params, void* hip_stream) {
+    API_BEGIN
+    vtx::HashObjectsArgs a{};
+    hash_args(a, particle_hash, particle_index, cell_start, cell_end, neighbors, positions, original_positions, params);
+    VTX_REQUIRE(vtx::hash_workspace_bytes(a.num_objects, a.table_size) <= (size_t)VTX_HASH_SCRATCH_BYTES,
+                VTX_ECAPACITY,
+                "num_objects / table_size need more workspace than the fixed scratch array holds "
+                "(40.5 B per object + 8 B per cell, at most 64 MiB)");
+    VTX_HIP(vtx::hash_objects_noalloc(a, static_cast<hipStream_t>(hip_stream)));
+    API_END
+}
+
 int vtx_set_timing(vtx_solver* s, int32_t on) {
     NEED(s);
     API_BEGIN s->engine.set_timing(on != 0); API_END

@@ -11,6 +11,8 @@
 // cub::DeviceRadixSort::SortPairs call (Velvet/SpatialHashGPU.cu:133-157).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "launch.hpp"
 #include "vec.hpp"
 #include "vtx_device.hpp"
@@ -695,6 +697,43 @@ hipError_t hash_objects(const HashObjectsArgs& a, void* workspace, hipStream_t s
     if (sb) hipLaunchKernelGGL(k_raw_split_cells, dim3(sb), dim3(VTX_TILE), 0, s, (const uint2*)cells, M, a.cell_start,
                                a.cell_end);
     return hipGetLastError();
+}
+
+// Workspace of hash_objects_noalloc: a fixed device array, one copy per GPU, alive until the
+// process ends. Host code never passes `scratch` itself to a kernel (on the host the name is a
+// shadow, not a device address): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[VTX_HASH_SCRATCH_BYTES];
+
+#define VTX_SCRATCH_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static void* scratch_ptr[VTX_SCRATCH_MAX_DEVICES];
+
+static hipError_t scratch_address(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= VTX_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t hash_objects_noalloc(const HashObjectsArgs& a, hipStream_t s) {
+    if (hash_workspace_bytes(a.num_objects, a.table_size) > (size_t)VTX_HASH_SCRATCH_BYTES) return hipErrorInvalidValue;
+    void* ws = nullptr;
+    const hipError_t e = scratch_address(&ws);
+    if (e != hipSuccess) return e;
+    // every workspace part is written before it is read (records, cell table fill, sort
+    // keys, histograms, scan partials), so nothing is cleared between calls
+    return hash_objects(a, ws, s);
 }
 
 // ---------------------------------------------------------------- host launchers

@@ -43,5 +43,9 @@ struct HashObjectsArgs {
 // Workspace bytes hash_objects needs for n objects and M cells (256-B aligned parts).
 size_t hash_workspace_bytes(uint32_t n, uint32_t M);
 hipError_t hash_objects(const HashObjectsArgs& a, void* workspace, hipStream_t s);
+// hash_objects with its workspace in a fixed per-device array of VTX_HASH_SCRATCH_BYTES (no
+// allocation); hipErrorInvalidValue when hash_workspace_bytes exceeds it.
+#define VTX_HASH_SCRATCH_BYTES (64u << 20)
+hipError_t hash_objects_noalloc(const HashObjectsArgs& a, hipStream_t s);
 
 }  // namespace vtx
