@@ -312,6 +312,9 @@ def main():
                     help="frames per variant for the launch-variant tuner before warm-up; 0 = defaults")
     ap.add_argument("--set-tuning", default="", metavar="K=V,...",
                     help="fix launch variants instead of tuning, e.g. use_graph=1,neighbor_order=1")
+    ap.add_argument("--reference-grid", action="store_true",
+                    help="keep every cloth on the reference hash grid (tune_hash_grid = 0); by default "
+                         "the cloths opt in and the tuner picks the hash-grid resolution")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-rates", default="50,100,200,400", help="comma-separated Poisson rates (req/s)")
@@ -333,6 +336,8 @@ def main():
         sys.exit(2)
 
     scenes, desc = workload_scenes(args.workload)
+    for sc in scenes:
+        sc.params.tune_hash_grid = 0 if args.reference_grid else 1
     tot = counts(scenes)
     substeps = scenes[0].params.num_substeps
     iters = scenes[0].params.num_iterations

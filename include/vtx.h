@@ -59,6 +59,10 @@ typedef struct vtx_params {
     float hash_cell_size_scalar;     /* 1.5: cell = diameter * scalar */
     int32_t hash_table_size;         /* extension: 0 = 2 x particles (reference); < 2^28 */
     float stretch_stiffness;         /* extension: lambda multiplier; 1 = reference */
+    int32_t tune_hash_grid;          /* extension: 0 = the reference's hash grid, never changed;
+                                        1 = vtx_tuning.hash_grid (set by hand or picked by
+                                        vtx_autotune) may shift this segment's grid origin and
+                                        enlarge its table (see vtx_tuning) */
 } vtx_params;
 
 /* Collider. Replaces the host Collider -> SDFCollider packing
@@ -78,7 +82,7 @@ typedef struct vtx_solver vtx_solver;
 
 /* Layout version of the structs above and below; vtx_abi_version() returns the library's, so a
  * binding can check at load time that it was written against the same layouts. */
-#define VTX_ABI_VERSION 3
+#define VTX_ABI_VERSION 4
 
 void vtx_params_default(vtx_params* p);
 const char* vtx_last_error(void);
@@ -151,8 +155,10 @@ int vtx_set_use_graph(vtx_solver* s, int32_t use_graph);
 int vtx_set_stencil_path(vtx_solver* s, int32_t enable);
 int vtx_stencil_segments(vtx_solver* s, int32_t* count);
 /* Extension (no reference counterpart; the reference has fixed launch shapes,
- * Velvet/Common.cuh:23-31): launch variants of one frame. Every combination gives bit-identical
- * results; only speed differs. */
+ * Velvet/Common.cuh:23-31, and a fixed hash grid, Velvet/SpatialHashGPU.hpp:21-22): launch
+ * variants of one frame and the hash-grid resolution of opted-in segments. Every combination of
+ * the launch variants gives bit-identical results; only speed differs. hash_grid is the
+ * exception (see below). */
 typedef struct vtx_tuning {
     int32_t use_graph;         /* replay frames from a hipGraph */
     int32_t neighbor_order;    /* neighbour search thread order: 0 particle index, 1 hash-sorted,
@@ -169,14 +175,29 @@ typedef struct vtx_tuning {
     int32_t hash_lds;          /* 1: self-colliding segments of at most 16384 particles (table of
                                   at most 2^16 cells) build their spatial hash in one workgroup;
                                   0: every segment takes the device-wide sort */
+    int32_t hash_grid;         /* hash-grid resolution of segments with tune_hash_grid = 1 (others
+                                  keep the reference grid): 0 = the reference grid (table of
+                                  hash_table_size cells, cell coordinates hashed as they are);
+                                  1, 2, 3 = cell coordinates shifted by VTX_HASH_GRID_SHIFT in
+                                  x, y and z before hashing, and a table of 1x, 2x, 4x
+                                  hash_table_size cells (kept below 2^28). The reference hash
+                                  maps cells (x, y, z) and (-x, y, -z) to one bucket, so a cloth
+                                  centred on the origin has every bucket shared by two distant
+                                  grid cells; the shift removes that. Unlike the other fields,
+                                  this one changes the hash tables and the order of neighbour
+                                  lists (not the set of neighbours within reach, except where
+                                  buckets collide), so results differ at rounding level. */
 } vtx_tuning;
+#define VTX_HASH_GRID_SHIFT 1000003   /* cells, added with 32-bit wrap-around; odd, so that no table
+                                         size (powers of two included) keeps the mirror pairing */
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out);   /* vtx_set_tuning ignores frame_ms */
 int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t);
 /* Time the variants on the current live set (coordinate descent, `frames` timed frames each,
  * positions and velocities restored after every trial, so the simulation does not advance),
  * apply the fastest and cache it for the live set's shape class (log2 buckets of particles and
  * segments, hash / stencil / dense-attachment presence, substeps x iterations). use_cache = 1
- * applies a class tuned earlier in this process without timing. */
+ * applies a class tuned earlier in this process without timing. hash_grid is tried only when a
+ * live segment has tune_hash_grid = 1. */
 int vtx_autotune(vtx_solver* s, int32_t frames, float frame_dt, int32_t use_cache, vtx_tuning* out);
 /* Bind the solver to an existing stream (hipStream_t), or NULL for the solver's own. */
 int vtx_set_stream(vtx_solver* s, void* hip_stream);

@@ -22,6 +22,7 @@ P = C.POINTER
 class OrcParams(C.Structure):
     _fields_ = [("num_substeps", i32), ("num_iterations", i32), ("max_num_neighbors", i32),
                 ("enable_self_collision", i32), ("interleaved_hash", i32), ("hash_table_size", i32),
+                ("hash_shift", i32),
                 ("max_speed", f64), ("gravity", f64 * 3), ("bend_compliance", f64),
                 ("damping", f64), ("relaxation_factor", f64), ("long_range_stretchiness", f64),
                 ("collision_margin", f64), ("friction", f64), ("particle_diameter", f64),
@@ -51,8 +52,8 @@ def lib():
             raise RuntimeError(f"oracle library missing: run `make -C {_HERE}`")
         L = C.CDLL(path)
         L.orc_simulate.argtypes = [P(OrcParams), P(OrcScene), i32, i32]
-        L.orc_hash64.argtypes = [i32, P(f64), P(f64), f64, f64, i32, i32, P(u32), P(u32), P(u32), P(u32)]
-        L.h32_hash.argtypes = [i32, P(f32), P(f32), f32, f32, i32, i32, P(u32), P(u32), P(u32),
+        L.orc_hash64.argtypes = [i32, P(f64), P(f64), f64, f64, i32, i32, i32, P(u32), P(u32), P(u32), P(u32)]
+        L.h32_hash.argtypes = [i32, P(f32), P(f32), f32, f32, i32, i32, i32, P(u32), P(u32), P(u32),
                                P(u32), P(u32), P(u32)]
         L.h32_hash_coords.argtypes = [i32, i32, i32, i32]
         L.h32_hash_coords.restype = i32
@@ -81,7 +82,9 @@ def _p(a, t):
     return a.ctypes.data_as(P(t))
 
 
-def make_params(prm, frame_dt=None) -> OrcParams:
+def make_params(prm, frame_dt=None, hash_grid=None) -> OrcParams:
+    """hash_grid = (table size, shift) replaces the reference grid (the parameters' table size,
+    no shift): the grid a tuned solver hashes with."""
     from velvet_amd import scenes as S
     o = OrcParams()
     o.num_substeps = prm.num_substeps
@@ -90,6 +93,9 @@ def make_params(prm, frame_dt=None) -> OrcParams:
     o.enable_self_collision = int(bool(prm.enable_self_collision))
     o.interleaved_hash = prm.interleaved_hash
     o.hash_table_size = prm.hash_table_size
+    o.hash_shift = 0
+    if hash_grid is not None:
+        o.hash_table_size, o.hash_shift = int(hash_grid[0]), int(hash_grid[1])
     o.max_speed = float(np.float32(prm.max_speed))
     for k in range(3):
         o.gravity[k] = float(np.float32(prm.gravity[k]))
@@ -120,7 +126,7 @@ def make_colliders(cols, frame_dt=None):
 class OracleSim:
     """FP64 run of one scene; keeps the arrays alive for the ctypes structs."""
 
-    def __init__(self, scene, frame_dt=None):
+    def __init__(self, scene, frame_dt=None, hash_grid=None):
         from velvet_amd import scenes as S
         self.scene = scene
         f = S.flatten(scene)
@@ -142,7 +148,7 @@ class OracleSim:
         self.tris = np.ascontiguousarray(f["tris"], dtype=np.uint32)
         self.normals = np.zeros_like(self.x)
         self.cols = make_colliders(scene.colliders, frame_dt)
-        self.params = make_params(scene.params, frame_dt)
+        self.params = make_params(scene.params, frame_dt, hash_grid)
         sc = OrcScene()
         sc.n = n
         sc.x, sc.v, sc.w, sc.x0 = _p(self.x, f64), _p(self.v, f64), _p(self.w, f64), _p(self.x0, f64)
@@ -162,13 +168,13 @@ class OracleSim:
         return self.x
 
 
-def simulate(scene, frames=None, max_substeps=-1, frame_dt=None):
-    s = OracleSim(scene, frame_dt)
+def simulate(scene, frames=None, max_substeps=-1, frame_dt=None, hash_grid=None):
+    s = OracleSim(scene, frame_dt, hash_grid)
     s.run(scene.frames if frames is None else frames, max_substeps)
     return s
 
 
-def hash64(p, x0, cell, diameter, M, max_nbr=64):
+def hash64(p, x0, cell, diameter, M, max_nbr=64, shift=0):
     p = np.ascontiguousarray(p, dtype=np.float64)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
     n = p.shape[0]
@@ -177,12 +183,13 @@ def hash64(p, x0, cell, diameter, M, max_nbr=64):
     si = np.empty(n, np.uint32)
     nb = np.full(max_nbr * n, 0xFFFFFFFF, np.uint32)
     lib().orc_hash64(n, _p(p, f64), _p(x0, f64), float(cell), float(diameter), int(M), int(max_nbr),
-                     _p(cs, u32), _p(ce, u32), _p(si, u32), _p(nb, u32))
+                     int(shift), _p(cs, u32), _p(ce, u32), _p(si, u32), _p(nb, u32))
     return cs, ce, si, nb.reshape(max_nbr, n)
 
 
-def hash32(p, x0, cell, diameter, M, max_nbr=64):
-    """binary32 hash stage: (particle_hash, sorted_hash, sorted_index, cell_start, cell_end, nbr)."""
+def hash32(p, x0, cell, diameter, M, max_nbr=64, shift=0):
+    """binary32 hash stage: (particle_hash, sorted_hash, sorted_index, cell_start, cell_end, nbr);
+    shift is added to every cell coordinate before hashing (0: the reference grid)."""
     p = np.ascontiguousarray(p, dtype=np.float32)
     x0 = np.ascontiguousarray(x0, dtype=np.float32)
     n = p.shape[0]
@@ -193,8 +200,8 @@ def hash32(p, x0, cell, diameter, M, max_nbr=64):
     ce = np.empty(M, np.uint32)
     nb = np.full(max_nbr * n, 0xFFFFFFFF, np.uint32)
     lib().h32_hash(n, _p(p, f32), _p(x0, f32), np.float32(cell), np.float32(diameter), int(M),
-                   int(max_nbr), _p(ph, u32), _p(sh, u32), _p(si, u32), _p(cs, u32), _p(ce, u32),
-                   _p(nb, u32))
+                   int(max_nbr), int(shift), _p(ph, u32), _p(sh, u32), _p(si, u32), _p(cs, u32),
+                   _p(ce, u32), _p(nb, u32))
     return ph, sh, si, cs, ce, nb.reshape(max_nbr, n)
 
 

@@ -32,19 +32,24 @@ int32_t h32_int_coord(float value, float cell) {
     return (int32_t)floorf(q);
 }
 
+/* the cell coordinate as hashed: shifted by `shift` cells with 32-bit wrap-around */
+static inline int32_t hashed_coord(float value, float cell, int32_t shift) {
+    return (int32_t)((uint32_t)h32_int_coord(value, cell) + (uint32_t)shift);
+}
+
 static inline float dist2(const float* a, const float* b) {
     float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 int h32_hash(int32_t n, const float* p, const float* x0, float cell, float diameter, int32_t M,
-             int32_t max_nbr, uint32_t* particle_hash, uint32_t* sorted_hash,
+             int32_t max_nbr, int32_t shift, uint32_t* particle_hash, uint32_t* sorted_hash,
              uint32_t* sorted_index, uint32_t* cell_start, uint32_t* cell_end, uint32_t* nbr) {
     uint32_t* cnt = (uint32_t*)calloc((size_t)M + 1, sizeof(uint32_t));
     if (!cnt) return -1;
     for (int32_t i = 0; i < n; i++) {
-        int32_t ix = h32_int_coord(p[3 * i], cell), iy = h32_int_coord(p[3 * i + 1], cell),
-                iz = h32_int_coord(p[3 * i + 2], cell);
+        int32_t ix = hashed_coord(p[3 * i], cell, shift), iy = hashed_coord(p[3 * i + 1], cell, shift),
+                iz = hashed_coord(p[3 * i + 2], cell, shift);
         particle_hash[i] = (uint32_t)h32_hash_coords(ix, iy, iz, M);
         cnt[particle_hash[i]]++;
     }
@@ -66,8 +71,8 @@ int h32_hash(int32_t n, const float* p, const float* x0, float cell, float diame
     for (int32_t id = 0; id < n; id++) {
         const float* pos = p + 3 * id;
         const float* org = x0 + 3 * id;
-        int32_t ix = h32_int_coord(pos[0], cell), iy = h32_int_coord(pos[1], cell),
-                iz = h32_int_coord(pos[2], cell);
+        int32_t ix = hashed_coord(pos[0], cell, shift), iy = hashed_coord(pos[1], cell, shift),
+                iz = hashed_coord(pos[2], cell, shift);
         int32_t k = 0;
         int full = 0;
         for (int32_t xx = ix - 1; xx <= ix + 1 && !full; xx++)

@@ -32,6 +32,8 @@ typedef struct orc_params {
     int32_t enable_self_collision;
     int32_t interleaved_hash;
     int32_t hash_table_size;          /* 0 -> 2 * n (Velvet/SpatialHashGPU.hpp:22) */
+    int32_t hash_shift;               /* added to every cell coordinate before hashing (32-bit
+                                         wrap-around); 0 is the reference grid */
     double max_speed;
     double gravity[3];
     double bend_compliance;
@@ -95,7 +97,7 @@ void orc_finalize(int32_t n, double* x, double* v, const double* p, double h, do
 void orc_normals(int32_t n, const double* x, int32_t T, const uint32_t* tris, double* normals);
 /* Spatial hash in FP64: cell tables and column-major neighbour lists (sentinel 0xffffffff). */
 int orc_hash64(int32_t n, const double* p, const double* x0, double cell, double diameter,
-               int32_t M, int32_t max_nbr, uint32_t* cell_start, uint32_t* cell_end,
+               int32_t M, int32_t max_nbr, int32_t shift, uint32_t* cell_start, uint32_t* cell_end,
                uint32_t* sorted_index, uint32_t* nbr);
 void orc_collide_particles(int32_t n, double* p, const double* x, const double* w,
                            const uint32_t* nbr, int32_t max_nbr, double diameter,
@@ -108,7 +110,7 @@ int orc_simulate(const orc_params* prm, orc_scene* sc, int32_t frames, int32_t m
 int32_t h32_hash_coords(int32_t x, int32_t y, int32_t z, int32_t M);
 int32_t h32_int_coord(float value, float cell);
 int h32_hash(int32_t n, const float* p, const float* x0, float cell, float diameter, int32_t M,
-             int32_t max_nbr, uint32_t* particle_hash, uint32_t* sorted_hash,
+             int32_t max_nbr, int32_t shift, uint32_t* particle_hash, uint32_t* sorted_hash,
              uint32_t* sorted_index, uint32_t* cell_start, uint32_t* cell_end, uint32_t* nbr);
 
 /* ---- CPU baseline (cpu_solver.c) ---- */

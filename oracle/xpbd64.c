@@ -309,16 +309,20 @@ static int32_t hash_coords(int32_t x, int32_t y, int32_t z, int32_t M) {
 
 /* ComputeParticleHash + stable sort + FindCellStart + CacheNeighbors in FP64
  * (Velvet/SpatialHashGPU.cu:34-130,159-196). */
+static int32_t coord64(double v, double cell, int32_t shift) {
+    return (int32_t)((uint32_t)(int32_t)floor(v / cell) + (uint32_t)shift);
+}
+
 int orc_hash64(int32_t n, const double* p, const double* x0, double cell, double diameter,
-               int32_t M, int32_t max_nbr, uint32_t* cell_start, uint32_t* cell_end,
+               int32_t M, int32_t max_nbr, int32_t shift, uint32_t* cell_start, uint32_t* cell_end,
                uint32_t* sorted_index, uint32_t* nbr) {
     int32_t* hash = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     uint32_t* cnt = (uint32_t*)calloc((size_t)M + 1, sizeof(uint32_t));
     if (!hash || !cnt) return -1;
     for (int32_t i = 0; i < n; i++) {
-        int32_t ix = (int32_t)floor(p[3 * i] / cell);
-        int32_t iy = (int32_t)floor(p[3 * i + 1] / cell);
-        int32_t iz = (int32_t)floor(p[3 * i + 2] / cell);
+        int32_t ix = coord64(p[3 * i], cell, shift);
+        int32_t iy = coord64(p[3 * i + 1], cell, shift);
+        int32_t iz = coord64(p[3 * i + 2], cell, shift);
         hash[i] = hash_coords(ix, iy, iz, M);
         cnt[hash[i]]++;
     }
@@ -336,8 +340,8 @@ int orc_hash64(int32_t n, const double* p, const double* x0, double cell, double
     const double cell2 = cell * cell, diam2 = diameter * diameter;
     for (int32_t id = 0; id < n; id++) {
         d3 pos = ld(p, id), org = ld(x0, id);
-        int32_t ix = (int32_t)floor(pos.x / cell), iy = (int32_t)floor(pos.y / cell),
-                iz = (int32_t)floor(pos.z / cell);
+        int32_t ix = coord64(pos.x, cell, shift), iy = coord64(pos.y, cell, shift),
+                iz = coord64(pos.z, cell, shift);
         int32_t k = 0;
         int full = 0;
         for (int32_t xx = ix - 1; xx <= ix + 1 && !full; xx++)
@@ -426,7 +430,8 @@ int orc_simulate(const orc_params* prm, orc_scene* sc, int32_t frames, int32_t m
             orc_predict(n, sc->x, sc->v, p, prm->gravity, h);
             if (prm->enable_self_collision) {
                 if (s % prm->interleaved_hash == 0)
-                    orc_hash64(n, p, sc->x0, prm->hash_cell_size, prm->particle_diameter, M, K, cs, ce, si, nbr);
+                    orc_hash64(n, p, sc->x0, prm->hash_cell_size, prm->particle_diameter, M, K, prm->hash_shift, cs,
+                               ce, si, nbr);
                 orc_collide_particles(n, p, sc->x, sc->w, nbr, K, prm->particle_diameter, prm->friction,
                                       prm->relaxation_factor);
             }

@@ -33,7 +33,7 @@ def test_abi_version_matches_header():
 
 
 def test_struct_sizes_match_header():
-    assert C.sizeof(_lib.VtxParams) == 4 * 20
+    assert C.sizeof(_lib.VtxParams) == 4 * 21
     assert C.sizeof(_lib.VtxCollider) == 4 * (1 + 3 + 3 + 16 + 16 + 1)
 
 

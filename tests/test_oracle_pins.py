@@ -13,6 +13,7 @@ import oracle
 from velvet_amd import scenes as S
 
 F32 = np.float32
+SENT = 0xFFFFFFFF
 
 
 def tiny_scene(x, stretch=(), bend=(), slots=(), attach=(), colliders=(), tris=None, **params):
@@ -317,6 +318,60 @@ def test_neighbour_sets_vs_brute_force():
     l64 = oracle.neighbor_lists(nb64)
     for i in range(n):
         assert set(l64[i]) == set(lists[i]) or len(lists[i]) == 64
+
+
+def test_hash_mirror_pairs_share_a_bucket():
+    """HashCoords gives (x, y, z) and (-x, y, -z) one bucket when x and z are odd: with odd
+    multipliers the products A = x*a and C = z*c are odd, -A = ~(A - 1) = ~(A ^ 1) mod 2^32, so
+    (-A) ^ B ^ (-C) = (A ^ 1) ^ B ^ (C ^ 1) = A ^ B ^ C. A cloth centred on the origin therefore
+    shares buckets with its own mirror image. Shifting every coordinate by an odd S first (the
+    tuned grid, S = 1000003) breaks the pairing, for power-of-two tables too: the two shifted
+    products differ by 2 S x a, which is not 0 modulo 4."""
+    h = oracle.lib().h32_hash_coords
+    rng = np.random.default_rng(3)
+    shift = 1000003
+    split = 0
+    for _ in range(300):
+        x, z = (2 * rng.integers(-(1 << 18), 1 << 18, 2) + 1).tolist()
+        y = int(rng.integers(-(1 << 18), 1 << 18))
+        for M in (7, 3200, 1 << 21):
+            assert h(x, y, z, M) == h(-x, y, -z, M), (x, y, z, M)
+        for M in (3200, 1 << 21):
+            split += h(x + shift, y + shift, z + shift, M) != h(-x + shift, y + shift, -z + shift, M)
+    assert split >= 590
+
+
+@pytest.mark.parametrize("mult", [2, 8])
+def test_shifted_grid_neighbour_sets(mult):
+    """On a shifted grid with a larger table the binary32 restatement finds the same neighbour
+    sets as on the reference grid and as an O(N^2) brute force (away from the radius), while its
+    buckets hold fewer foreign candidates."""
+    x0, p, c = folded_cloth(n=24, seed=5)
+    n = p.shape[0]
+    cell = float(F32(F32(c.diameter) * F32(1.5)))
+    ref = oracle.hash32(p, x0, cell, float(c.diameter), 2 * n)
+    got = oracle.hash32(p, x0, cell, float(c.diameter), mult * n, shift=1000003)
+    lr, lg = oracle.neighbor_lists(ref[5]), oracle.neighbor_lists(got[5])
+    truth = _brute_force_sets(p, x0, cell, float(c.diameter))
+    for i in range(n):
+        assert set(lg[i]) == set(lr[i]), i
+        extra = truth[i] ^ set(lg[i])
+        if extra:
+            d = np.linalg.norm(p[i].astype(np.float64) - p[list(extra)], axis=1)
+            assert np.all(d > 0.999 * cell), (i, extra)
+    # candidates walked: the occupants of the 27 buckets visited, per particle
+    def walked(cs, ce, M, shift):
+        cnt = np.where(cs == SENT, 0, ce.astype(np.int64) - cs.astype(np.int64))
+        tot = 0
+        for i in range(n):
+            ix, iy, iz = (oracle.lib().h32_int_coord(F32(v), F32(cell)) + shift for v in p[i])
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        tot += min(64, cnt[oracle.lib().h32_hash_coords(ix + dx, iy + dy, iz + dz, M)])
+        return tot / n
+    w_ref, w_got = walked(ref[3], ref[4], 2 * n, 0), walked(got[3], got[4], mult * n, 1000003)
+    assert w_got < 0.8 * w_ref, (w_ref, w_got)
 
 
 def test_cell_tables_vs_stable_argsort():

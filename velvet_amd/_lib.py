@@ -17,7 +17,7 @@ i32, u32, f32 = C.c_int32, C.c_uint32, C.c_float
 P = C.POINTER
 
 
-ABI_VERSION = 3   # VTX_ABI_VERSION of include/vtx.h for the structures below
+ABI_VERSION = 4   # VTX_ABI_VERSION of include/vtx.h for the structures below
 
 
 class VtxParams(C.Structure):
@@ -27,7 +27,7 @@ class VtxParams(C.Structure):
                 ("collision_margin", f32), ("friction", f32), ("enable_self_collision", i32),
                 ("interleaved_hash", i32), ("particle_diameter", f32),
                 ("particle_diameter_scalar", f32), ("hash_cell_size_scalar", f32),
-                ("hash_table_size", i32), ("stretch_stiffness", f32)]
+                ("hash_table_size", i32), ("stretch_stiffness", f32), ("tune_hash_grid", i32)]
 
 
 class VtxCollider(C.Structure):
@@ -44,7 +44,7 @@ class VtxServerConfig(C.Structure):
 class VtxTuning(C.Structure):
     _fields_ = [("use_graph", i32), ("neighbor_order", i32), ("lra_batch", i32), ("xcd_remap", i32),
                 ("frame_ms", f32), ("gather_block", i32), ("streams", i32), ("lra_split", i32),
-                ("hash_lds", i32)]
+                ("hash_lds", i32), ("hash_grid", i32)]
 
 
 class VtxHashParams(C.Structure):

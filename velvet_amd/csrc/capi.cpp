@@ -76,6 +76,7 @@ void vtx_params_default(vtx_params* p) {
     p->hash_cell_size_scalar = 1.5f;
     p->hash_table_size = 0;
     p->stretch_stiffness = 1.f;
+    p->tune_hash_grid = 0;
 }
 
 const char* vtx_last_error(void) { return g_error.c_str(); }
@@ -216,6 +217,7 @@ static void to_abi(const vtx::Tuning& t, double ms, vtx_tuning* out) {
     out->streams = t.streams;
     out->lra_split = t.lra_split ? 1 : 0;
     out->hash_lds = t.hash_lds ? 1 : 0;
+    out->hash_grid = t.hash_grid;
 }
 
 int vtx_get_tuning(vtx_solver* s, vtx_tuning* out) {
@@ -234,6 +236,7 @@ int vtx_set_tuning(vtx_solver* s, const vtx_tuning* t) {
     k.gather_block = t->gather_block;
     k.streams = t->streams;
     k.lra_split = t->lra_split != 0;
+    k.hash_grid = t->hash_grid;
     k.hash_lds = t->hash_lds != 0;
     s->engine.set_tuning(k);
     API_END
@@ -441,6 +444,7 @@ int vtx_hash_objects(uint32_t* particle_hash, uint32_t* particle_index, uint32_t
 }
 
 static_assert(VTX_HASH_SCRATCH_BYTES == VTX_HASH_SCRATCH_LIMIT, "scratch size in launch.hpp and vtx.h differ");
+static_assert(VTX_HASH_SHIFT == VTX_HASH_GRID_SHIFT, "grid shift in vtx_device.hpp and vtx.h differ");
 
 int vtx_hash_objects_noalloc(uint32_t* particle_hash, uint32_t* particle_index, uint32_t* cell_start,
                              uint32_t* cell_end, uint32_t* neighbors, const float* positions,

@@ -305,6 +305,7 @@ static void validate_params(const vtx_params& p) {
                 "max_num_neighbors must be in [1, 64]");
     VTX_REQUIRE(p.interleaved_hash >= 1, VTX_EINVAL, "interleaved_hash must be >= 1");
     VTX_REQUIRE(p.hash_table_size >= 0, VTX_EINVAL, "hash_table_size must be >= 0");
+    VTX_REQUIRE(p.tune_hash_grid == 0 || p.tune_hash_grid == 1, VTX_EINVAL, "tune_hash_grid must be 0 or 1");
 }
 
 int32_t Engine::add_segment(const vtx_params& p, bool active) {
@@ -851,7 +852,7 @@ static Ranges needs_of(const Segment& sg) {
     const Prepared& pr = sg.prep;
     Ranges r;
     r.n_p = sg.n();
-    r.n_cell = sg.selfcol() ? sg.table_size() : 0;
+    r.n_cell = sg.cell_capacity();
     r.n_nbr = sg.selfcol() ? (size_t)sg.prm.max_num_neighbors * sg.n() : 0;
     r.n_st = pr.est.size();
     r.n_at = pr.eat.size();
@@ -1205,6 +1206,7 @@ void Engine::rebuild_tables() {
     any_lra_dense_ = false;
     any_lra_split_ = false;
     any_hash_lds_ = false;
+    any_grid_tunable_ = false;
     need_collide_ = false;
     live_particles_ = 0;
     uint32_t maxbits = 0, hbase = 0, compact = 0;
@@ -1245,6 +1247,15 @@ void Engine::rebuild_tables() {
         cum += sg.n();
         const uint32_t ti = (uint32_t)hseg.size();
         table_seg_.push_back((int32_t)si);
+        if (sg.dev.flags & VTX_SEG_SELFCOL) {   // hash grid of this table set (Tuning::hash_grid)
+            const bool shifted = sg.prm.tune_hash_grid && tun_.hash_grid > 0;
+            any_grid_tunable_ = any_grid_tunable_ || sg.prm.tune_hash_grid;
+            sg.dev.M = sg.grid_table_size(shifted ? tun_.hash_grid : 0);
+            const uint64_t c = UINT64_MAX / sg.dev.M + 1;   // ceil(2^64 / M); wraps to 0 for M = 1
+            sg.dev.mod_lo = (uint32_t)(c & 0xFFFFFFFFu);
+            sg.dev.mod_hi = (uint32_t)(c >> 32);
+            sg.dev.flags = (sg.dev.flags & ~VTX_SEG_HASH_SHIFT) | (shifted ? VTX_SEG_HASH_SHIFT : 0u);
+        }
         const bool hlds = (sg.dev.flags & VTX_SEG_SELFCOL) && sg.dev.n <= VTX_HASH_LDS_N && sg.dev.M <= (1u << 16);
         any_hash_lds_ = any_hash_lds_ || hlds;
         const bool in_lds = hlds && tun_.hash_lds;
@@ -1656,7 +1667,10 @@ void Engine::set_tuning(const Tuning& t) {
     VTX_REQUIRE(t.gather_block == 64 || t.gather_block == 128 || t.gather_block == 256, VTX_EINVAL,
                 "gather_block must be 64, 128 or 256");
     VTX_REQUIRE(t.streams >= 1 && t.streams <= VTX_MAX_STREAMS, VTX_EINVAL, "streams must be 1 to 4");
-    if (t.streams != tun_.streams || t.lra_split != tun_.lra_split || t.hash_lds != tun_.hash_lds) tables_dirty_ = true;
+    VTX_REQUIRE(t.hash_grid >= 0 && t.hash_grid <= 3, VTX_EINVAL, "hash_grid must be 0 to 3");
+    if (t.streams != tun_.streams || t.lra_split != tun_.lra_split || t.hash_lds != tun_.hash_lds ||
+        t.hash_grid != tun_.hash_grid)
+        tables_dirty_ = true;
     tun_ = t;
     ds_.remap = t.remap ? 1u : 0u;
     for (DevState& g : groups_) g.remap = ds_.remap;
@@ -1676,7 +1690,8 @@ std::string Engine::shape_class() {
     std::ostringstream o;
     const uint32_t np = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, live_particles_), UINT32_MAX);
     o << "p" << ceil_log2(np) << ".s" << ceil_log2(std::max(1u, nseg))
-      << ".h" << (any_selfcol_ ? 1 : 0) << ".g" << (ngrid ? 1 : 0) << ".l" << (any_lra_split_ ? 2 : any_lra_dense_ ? 1 : 0) << "."
+      << ".h" << (any_selfcol_ ? (any_grid_tunable_ ? 2 : 1) : 0) << ".g" << (ngrid ? 1 : 0) << ".l"
+      << (any_lra_split_ ? 2 : any_lra_dense_ ? 1 : 0) << "."
       << substeps_ << "x" << iterations_;
     return o.str();
 }
@@ -1756,15 +1771,18 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
             return (double)ms / frames;
         };
         bms = trial(best);
-        // one knob at a time, every other value of it against the best so far
-        for (int knob = 0; knob < 8; knob++) {
+        // one knob at a time, every other value of it against the best so far; the hash grid
+        // (opted-in segments only) goes before the knobs of the hash and neighbour phases
+        static const int order[] = {0, 8, 1, 2, 3, 4, 5, 6, 7};
+        for (int knob : order) {
+            if (knob == 8 && !any_grid_tunable_) continue;
             if (knob == 1 && !any_selfcol_) continue;
             if (knob == 2 && !any_lra_dense_) continue;
             if (knob == 4 && !need_collide_) continue;
             if (knob == 5 && table_seg_.size() < 2) continue;
             if (knob == 6 && !any_lra_split_) continue;
             if (knob == 7 && !any_hash_lds_) continue;
-            const int nval = knob == 5 ? VTX_MAX_STREAMS : knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
+            const int nval = knob == 5 || knob == 8 ? 4 : knob == 1 || knob == 2 || knob == 4 ? 3 : 2;
             const Tuning base = best;
             for (int v = 0; v < nval; v++) {
                 Tuning t = base;
@@ -1776,7 +1794,9 @@ Tuning Engine::autotune(int frames, float frame_dt, bool use_cache, double* fram
                 if (knob == 5) t.streams = v + 1;
                 if (knob == 6) t.lra_split = v != 0;
                 if (knob == 7) t.hash_lds = v != 0;
-                if (knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
+                if (knob == 8) t.hash_grid = v;
+                if (knob == 8 ? t.hash_grid == base.hash_grid
+                    : knob == 0 ? t.graph == base.graph : knob == 1 ? t.nbr_order == base.nbr_order
                                 : knob == 2 ? t.lra_batch == base.lra_batch
                                 : knob == 3 ? t.remap == base.remap
                                 : knob == 4 ? t.gather_block == base.gather_block

@@ -116,6 +116,16 @@ struct Segment {
     uint32_t n() const { return (uint32_t)w.size(); }
     uint32_t table_size() const { return prm.hash_table_size > 0 ? (uint32_t)prm.hash_table_size : 2u * n(); }
     bool selfcol() const { return prm.enable_self_collision && n() > 0; }
+    // Hash table cells of the grid for a Tuning::hash_grid value: the parameters' table size,
+    // enlarged 2x / 4x at values 2 / 3 for segments that opted in (kept below 2^28).
+    uint32_t grid_table_size(int hash_grid) const {
+        uint64_t m = table_size();
+        if (prm.tune_hash_grid && hash_grid >= 2) m <<= (hash_grid - 1);
+        while (m >= (1u << 28) && m > table_size()) m >>= 1;
+        return (uint32_t)m;
+    }
+    // cells reserved in the pool: room for every hash_grid value
+    uint32_t cell_capacity() const { return selfcol() ? grid_table_size(3) : 0u; }
 };
 
 // Launch variants the tuner chooses among. Every combination gives bit-identical results: they
@@ -132,6 +142,9 @@ struct Tuning {
     bool lra_split = true;    // grid segments with >= VTX_LRA_SPLIT_MIN dense slots: four threads
                               // per particle in the Jacobi pass
     bool hash_lds = true;     // small self-colliding segments build their hash in one workgroup
+    int hash_grid = 0;        // hash-grid resolution of segments with tune_hash_grid: 0 the
+                              // reference grid; 1..3 origin shifted, table 1x / 2x / 4x. The only
+                              // field that changes results (list order, bucket sharing)
 };
 
 // A timed launch group: its kernel and the reference's timer labels of the stages it runs
@@ -303,6 +316,7 @@ private:
     bool any_lra_dense_ = false;
     bool any_lra_split_ = false;   // a stencil segment has >= VTX_LRA_SPLIT_MIN dense slots
     bool any_hash_lds_ = false;    // a self-colliding segment is small enough for k_hash_small
+    bool any_grid_tunable_ = false;   // a live self-colliding segment has tune_hash_grid
     bool use_stencil_ = true;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = -1.f;

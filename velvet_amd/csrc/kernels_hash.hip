@@ -40,6 +40,16 @@ __device__ __forceinline__ int32_t int_coord(float v, float cell) {
     return (int32_t)floorf(__fdiv_rn(v, cell));
 }
 
+// Cell coordinate as hashed: ComputeIntCoord plus the segment's grid shift (0 on the reference
+// grid, VTX_HASH_SHIFT on a tuned one), with 32-bit wrap-around.
+__device__ __forceinline__ int32_t hash_coord(float v, float cell, uint32_t shift) {
+    return (int32_t)((uint32_t)int_coord(v, cell) + shift);
+}
+
+__device__ __forceinline__ uint32_t seg_shift(const SegDev& sg) {
+    return (sg.flags & VTX_SEG_HASH_SHIFT) ? VTX_HASH_SHIFT : 0u;
+}
+
 // ComputeParticleHash_Kernel (Velvet/SpatialHashGPU.cu:34-42) over predicted positions.
 __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
     const TileDev t = d.tiles[tile_of(d.remap)];
@@ -53,8 +63,9 @@ __global__ __launch_bounds__(VTX_TILE) void k_hash_keys(DevState d, int in) {
         return;
     }
     const float4 p = d.P[in][g];
-    d.keys[0][g] = hash_coords(int_coord(p.x, sg.cell), int_coord(p.y, sg.cell),
-                               int_coord(p.z, sg.cell), sg.M, seg_mod(sg));
+    const uint32_t sh = seg_shift(sg);
+    d.keys[0][g] = hash_coords(hash_coord(p.x, sg.cell, sh), hash_coord(p.y, sg.cell, sh),
+                               hash_coord(p.z, sg.cell, sh), sg.M, seg_mod(sg));
 }
 
 // Digit histogram of one sort tile (count keys from `keys`): the count of digit g goes to
@@ -270,6 +281,7 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
     uint2* __restrict__ cells = d.cells + sg.tbase;
     for (uint32_t i = tid; i < M; i += VTX_HASH_LDS_BT) cells[i] = make_uint2(VTX_INVALID, VTX_INVALID);
     const uint64_t mc = seg_mod(sg);
+    const uint32_t sh = seg_shift(sg);
     const float4* __restrict__ P = d.P[in] + sg.pbase;
     constexpr uint32_t U = 8, STEP = U * VTX_HASH_LDS_BT;
     for (uint32_t i0 = tid; i0 < n; i0 += STEP) {
@@ -283,8 +295,8 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
             if (i < n)
-                buf[0][i] = hash_coords(int_coord(p[u].x, sg.cell), int_coord(p[u].y, sg.cell),
-                                        int_coord(p[u].z, sg.cell), M, mc) << 16 | i;
+                buf[0][i] = hash_coords(hash_coord(p[u].x, sg.cell, sh), hash_coord(p[u].y, sg.cell, sh),
+                                        hash_coord(p[u].z, sg.cell, sh), M, mc) << 16 | i;
         }
     }
     __syncthreads();
@@ -422,18 +434,19 @@ __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
 // The 27-cell walk of one particle (CacheNeighbors_Kernel, Velvet/SpatialHashGPU.cu:94-129): p / o
 // its predicted / initial position, id its index, sP the 32-B sorted records (position with
 // w = index at 2i, initial position at 2i + 1), cells the (start, end) table of M cells, nb its
-// column (entry k at nb[k * n]). crange is the workgroup's LDS cache of cell ranges, one
-// column per thread.
+// column (entry k at nb[k * n]), shift the grid shift of the cell coordinates. crange is the
+// workgroup's LDS cache of cell ranges, one column per thread.
 template <int BS>
 __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, const uint32_t id,
                                               const float4* __restrict__ sP, const uint2* __restrict__ cells,
                                               uint32_t* __restrict__ nb, const uint32_t n, const float cell,
                                               const float cell2, const float diam2, const uint32_t M,
-                                              const uint64_t mc, const uint32_t K,
+                                              const uint64_t mc, const uint32_t K, const uint32_t shift,
                                               uint32_t (&crange)[VTX_NB_CACHE][BS]) {
 #pragma clang fp contract(off)
     const float4* __restrict__ sX0 = sP + 1;
-    const int32_t ix = int_coord(p.x, cell), iy = int_coord(p.y, cell), iz = int_coord(p.z, cell);
+    const int32_t ix = hash_coord(p.x, cell, shift), iy = hash_coord(p.y, cell, shift),
+                  iz = hash_coord(p.z, cell, shift);
     uint32_t mask = 0, total = 0, nz = 0;
     // VTX_NB_BATCH cell ranges in flight at a time, each one 8-byte load at a 32-bit byte offset
     // (tables have M < 2^28 cells)
@@ -547,7 +560,8 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const float4 o = sorted ? sX0[2 * i] : d.X0[sg.pbase + i];
     const uint32_t id = sorted ? __float_as_uint(p.w) : i;
     uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    neighbor_walk<BS>(p, o, id, sP, cells, nb, n, sg.cell, sg.cell2, sg.diam2, sg.M, seg_mod(sg), sg.max_nbr, crange);
+    neighbor_walk<BS>(p, o, id, sP, cells, nb, n, sg.cell, sg.cell2, sg.diam2, sg.M, seg_mod(sg), sg.max_nbr,
+                      seg_shift(sg), crange);
 }
 
 // ---------------------------------------------------------------- kernel-launch tier
@@ -623,7 +637,7 @@ __global__ __launch_bounds__(BS) void k_raw_neighbors(const float4* __restrict__
     if (i >= n) return;
     const float4 p = rec[2 * (size_t)i], o = rec[2 * (size_t)i + 1];
     const uint32_t id = __float_as_uint(p.w);
-    neighbor_walk<BS>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, crange);
+    neighbor_walk<BS>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, 0u, crange);
 }
 
 // The interleaved table to the caller's cell_start / cell_end (empty cells: ~0u in both).

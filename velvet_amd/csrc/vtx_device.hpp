@@ -40,6 +40,13 @@
 // kernels skip it.
 #define VTX_SEG_HASH_LDS 1024u
 #define VTX_HASH_LDS_N 16384
+// The segment hashes its cell coordinates after adding VTX_HASH_SHIFT to each (32-bit
+// wrap-around): a tuned hash grid. HashCoords gives cells (x, y, z) and (-x, y, -z) one bucket
+// (with odd x and z the two products' two's-complement negations cancel in the XOR, and abs()
+// folds the remaining cases), so a cloth centred on the origin shares every bucket with its
+// mirror image; shifted coordinates have no such partner nearby.
+#define VTX_SEG_HASH_SHIFT 2048u
+#define VTX_HASH_SHIFT 1000003u
 // A particle's attachment terms are summed in four parts by slot index -- slots
 // [q*Qs, (q+1)*Qs) with Qs = ceil(scount / 4) -- each part from zero in slot order, and the parts
 // are added to the particle's delta in order q = 0..3. One lane can run all four parts, or four

@@ -65,8 +65,9 @@ def uniform(seed: int, stream: int, count: int, lo: float = 0.0, hi: float = 1.0
 class Params:
     """Solver knobs; field names and defaults follow VtSimParams (Velvet/Common.hpp:19-48).
 
-    ``hash_table_size`` (0 = 2 x particles, Velvet/SpatialHashGPU.hpp:22) and
-    ``stretch_stiffness`` (a multiplier on the stretch lambda; 1.0 is the reference) are
+    ``hash_table_size`` (0 = 2 x particles, Velvet/SpatialHashGPU.hpp:22),
+    ``stretch_stiffness`` (a multiplier on the stretch lambda; 1.0 is the reference) and
+    ``tune_hash_grid`` (1 lets the tuner shift the hash grid and enlarge the table) are
     extensions.
     """
     num_substeps: int = 2
@@ -87,6 +88,7 @@ class Params:
     hash_cell_size_scalar: float = 1.5
     hash_table_size: int = 0
     stretch_stiffness: float = 1.0
+    tune_hash_grid: int = 0
 
 
 # ---------------------------------------------------------------------------------------------

@@ -35,7 +35,7 @@ def to_vtx_params(p: S.Params) -> VtxParams:
     v = VtxParams()
     lib().vtx_params_default(C.byref(v))
     for name in ("num_substeps", "num_iterations", "max_num_neighbors", "interleaved_hash",
-                 "hash_table_size"):
+                 "hash_table_size", "tune_hash_grid"):
         setattr(v, name, int(getattr(p, name)))
     v.enable_self_collision = int(bool(p.enable_self_collision))
     for name in ("max_speed", "bend_compliance", "damping", "relaxation_factor",
@@ -200,9 +200,10 @@ class ClothSolver:
     def set_use_graph(self, on: bool):
         check(lib().vtx_set_use_graph(self.h, int(bool(on))))
 
-    # Launch variants (all give bit-identical results) and the tuner that picks among them.
+    # Launch variants (all give bit-identical results), the hash grid of segments that opted in
+    # (tune_hash_grid), and the tuner that picks among them.
     _TUNING_KEYS = ("use_graph", "neighbor_order", "lra_batch", "xcd_remap", "gather_block", "streams", "lra_split",
-                    "hash_lds")
+                    "hash_lds", "hash_grid")
 
     def tuning(self) -> dict:
         t = VtxTuning()
