@@ -474,28 +474,33 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
         nb[0] = VTX_INVALID;
         return;
     }
-    // position of the next candidate (called only while candidates remain); moving on to the
-    // next non-empty cell is branch-free unless that cell lies beyond the LDS cache
-    uint32_t z = 0, s = 0, left = 0;
+    // position of the next candidate (called only while candidates remain): s runs through the
+    // current cell's range up to e; there the walk moves on to the next non-empty cell, from
+    // the LDS cache or, beyond it, from the cell table again
+    uint32_t z = 0, s = 0, e = 0;
     auto next = [&]() -> uint32_t {
-        const bool sw = left == 0;
-        const uint32_t e = crange[z < VTX_NB_CACHE ? z : VTX_NB_CACHE - 1u][threadIdx.x];
-        uint32_t st = e & 0x1FFFFFFu, ln = e >> 25;
-        if (sw && z >= VTX_NB_CACHE) {
-            const int q = __builtin_ctz(mask);
-            const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, M, mc)];
-            st = c.x;
-            ln = min(c.y, c.x + K) - c.x;
+        if (s == e) {
+            uint32_t st, ln;
+            if (z < VTX_NB_CACHE) {
+                const uint32_t c = crange[z][threadIdx.x];
+                st = c & 0x1FFFFFFu;
+                ln = c >> 25;
+            } else {
+                const int q = __builtin_ctz(mask);
+                const uint2 c = cells[hash_coords(ix - 1 + q / 9, iy - 1 + (q / 3) % 3, iz - 1 + q % 3, M, mc)];
+                st = c.x;
+                ln = min(c.y, c.x + K) - c.x;
+            }
+            mask &= mask - 1u;
+            z++;
+            s = st;
+            e = st + ln;
         }
-        mask = sw ? (mask & (mask - 1u)) : mask;
-        z += sw ? 1u : 0u;
-        s = sw ? st : s;
-        left = (sw ? ln : left) - 1u;
         return s++;
     };
-    // candidate test in walk order, one predicate and one predicated store; a full list takes
-    // no more entries
-    uint32_t k = 0;
+    // candidate test in walk order, one predicate and one predicated store at the running
+    // offset koff = k n; a full list takes no more entries
+    uint32_t k = 0, koff = 0;
     auto take = [&](float4 qp, float4 r) {
         const uint32_t j = __float_as_uint(qp.w);
         const float dx = p.x - qp.x, dy = p.y - qp.y, dz = p.z - qp.z;
@@ -503,8 +508,9 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         const bool acc = (j != id) & (d2 < cell2) & (e2 > diam2) & (k < K);
-        if (acc) nb[(size_t)k * n] = j;
+        if (acc) nb[koff] = j;
         k += acc ? 1u : 0u;
+        koff += acc ? n : 0u;
     };
     // two candidates per trip with alternating load buffers: the loads of the next candidate
     // are in flight while the current one is tested
@@ -526,7 +532,7 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
         }
         take(qb, rb);
     }
-    if (k < K) nb[(size_t)k * n] = VTX_INVALID;                  // a full list has no sentinel
+    if (k < K) nb[koff] = VTX_INVALID;                           // a full list has no sentinel
 }
 
 template <int ORDER, int BS>
