@@ -9,9 +9,14 @@ search order, attachment load batch and XCD remap (all bit-identical). Plain = o
 cloth, each launching its own kernels per stage on one shared stream, no graph.
 
 A step is one 1/60 s frame of every cloth. ``value`` = sum(particles) x substeps x steps /
-seconds of the timed region (inputs resident in HBM). The line also carries the plain path's
-rate, per-frame latency percentiles, the dominant kernel's roofline (HIP events on the
-solver's stream) and the CPU baseline (the restated reference CPU solver on host cores).
+seconds of the timed region (inputs resident in HBM); the timed frames are the last ones of the
+workload's 120-frame horizon, after the cloths have settled on their colliders. The cloths opt
+in to the tuned hash grid (``--reference-grid`` keeps the reference's). The line also carries
+the plain path's rate, per-frame latency percentiles, the dominant kernel's roofline (HIP events
+on the solver's stream; algorithmic bytes with dense long-range attachments at their 4-B
+distance, beside the reference-layout figure, and the copy-calibrated counter bytes from
+``profiles/``), the CPU baseline (the restated reference CPU solver on host cores), the serving
+sweep, and the same measurements for the single-cloth configurations 1-3 (``configs``).
 """
 from __future__ import annotations
 
@@ -135,13 +140,14 @@ def plain_rate(scenes, frames, stream):
     return dt
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of the plain Jacobi pass from the committed, copy-calibrated
-    FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs), or None."""
+def measured_traffic(workload, row_prefix="| `void k_iterate_grid<0,"):
+    """HBM bytes per launch of a kernel (the plain Jacobi pass by default) from the committed,
+    copy-calibrated FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs),
+    or None."""
     path = os.path.join(ROOT, "profiles", f"{workload}_hbm_counters.md")
     try:
         for ln in open(path):
-            if ln.startswith("| `void k_iterate_grid<0,"):
+            if ln.startswith(row_prefix):
                 cells = [c.strip() for c in ln.strip().strip("|").split("|")]
                 return (float(cells[1]) + float(cells[2])) * 1e6, os.path.relpath(path, ROOT)
     except OSError:
@@ -149,14 +155,88 @@ def measured_traffic(workload):
     return None, None
 
 
-def cpu_baseline(scenes, budget_s):
+def jacobi_bytes(scenes):
+    """Algorithmic bytes of one Jacobi pass over the workload: the reference-layout figure
+    12S + 20B + 12A + 12L + 28N, except that a cloth whose attachments are the dense long-range
+    form (every particle to every slot, in slot order) needs only the 4-B distance per
+    attachment (particle and slot follow from the position in the table)."""
+    tot = 0
+    for sc in scenes:
+        c = sc.counts()
+        dense = all(cl.attach_pid.shape[0] > 0 and cl.attach_pid.shape[0] == cl.slots.shape[0] * cl.n
+                    for cl in sc.cloths) and c["A"] > 0
+        tot += 12 * c["S"] + 20 * c["B"] + (4 if dense else 12) * c["A"] + 12 * c["L"] + 28 * c["N"]
+    return tot
+
+
+def dominant(kt):
+    """(label, entry) of the launch group with the largest total time in a timing report."""
+    items = [(k, v) for k, v in kt.items() if k != "total" and v.get("launches")]
+    return max(items, key=lambda kv: kv[1]["ms"])
+
+
+def config_leg(name, steps, warmup, tune_frames, cpu_seconds):
+    """One of the single-cloth configurations: tuned, settled so that the timed frames end at
+    the configuration's horizon, timed, per-stage times, the dominant kernel's roofline and a
+    bounded sample of the restated reference CPU solver."""
+    scenes, desc = workload_scenes(name)
+    horizon = scenes[0].frames
+    tot = counts(scenes)
+    substeps = scenes[0].params.num_substeps
+    for sc in scenes:
+        sc.params.tune_hash_grid = 1
+    solver = ClothSolver(use_graph=True)
+    for sc in scenes:
+        solver.load_scene(sc)
+    tuning = solver.autotune(frames=tune_frames) if tune_frames > 0 else solver.tuning()
+    settle = max(0, horizon - warmup - steps)
+    for _ in range(settle + warmup):
+        solver.Simulate()
+    dt = time_frames(solver, steps)
+    rate = tot["N"] * substeps * steps / dt
+    kt = kernel_times(solver, 3)
+    solver.close()
+    label, ent = dominant(kt)
+    avg_ms = ent["ms"] / ent["launches"]
+    out = {"workload": f"{name}: {desc}", "particles": tot["N"], "value": rate, "unit": "particle-substeps/s",
+           "ms_per_step": dt / steps * 1e3, "frames_timed": [settle + warmup + 1, settle + warmup + steps],
+           "tuning": tuning, "dominant": {"label": label, "kernel": ent["kernel"], "avg_launch_ms": round(avg_ms, 6),
+                                          "launches_per_frame": ent["launches"] / 3},
+           "kernels_ms": kt}
+    if ent["kernel"] == "k_iterate":
+        alg = jacobi_bytes(scenes)
+        ach = alg / (avg_ms * 1e-3) / 1e9
+        traffic, src = measured_traffic(name)
+        out["roofline"] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
+                           "algorithmic_bytes_per_launch": alg,
+                           "reference_layout_bytes_per_launch": S.algorithmic_bytes_per_iteration(tot)}
+        if traffic:
+            out["roofline"]["traffic_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    else:
+        out["roofline"] = {"bound": "launch", "note": f"{ent['launches'] / 3:.0f} launches of {ent['kernel']} per "
+                                                      f"frame average {avg_ms * 1e3:.1f} us; the frame is launch "
+                                                      f"bound, not HBM bound"}
+    if cpu_seconds > 0:
+        rate_c, spent, frames, _ = cpu_baseline(scenes, cpu_seconds, warm=tot["N"] < 500_000)
+        out["cpu_baseline"] = {"value": rate_c, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+                               "sample": f"restated reference CPU solver, single thread, {frames} timed frame(s) of "
+                                         f"this configuration, {spent:.1f} s"}
+    return out
+
+
+def cpu_baseline(scenes, budget_s, warm=True):
+    """The restated reference CPU solver on one thread: one timed frame per cloth (after a
+    warm-up frame unless warm is False, for cloths whose single frame fills the budget), cloths
+    in turn until the budget is spent."""
     import oracle
     done_ps, spent, frames, cloths = 0, 0.0, 0, 0
     k = 0
     while spent < budget_s and k < 4 * len(scenes):
         sc = scenes[k % len(scenes)]
         cpu = oracle.CpuSolver(sc)
-        cpu.simulate()                     # warm-up frame, excluded
+        if warm:
+            cpu.simulate()                 # warm-up frame, excluded
         t0 = time.perf_counter()
         cpu.simulate()
         dt = time.perf_counter() - t0
@@ -216,7 +296,10 @@ def serve_pool(seed, count=64):
     """Job descriptions (scene, horizon) the serving run's requests cycle through: sizes
     log-uniform 32^2..512^2, horizons U(30, 600), attachments and colliders as drawn per job.
     Building a cloth is the client's work, so it happens before the timed run."""
-    return [S.serving_job(seed, k) for k in range(count)]
+    pool = [S.serving_job(seed, k) for k in range(count)]
+    for sc, _ in pool:
+        sc.params.tune_hash_grid = 1       # the server's tuning picks the grid (hash_grid)
+    return pool
 
 
 def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms=0.0, ahead=4,
@@ -229,13 +312,16 @@ def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms
     latency is the server's own wall time per step;
     admission wait is the time from a request's arrival to the end of the step that admitted it
     (requests still waiting at the end are counted in `backlog`, not in the wait percentiles).
-    Finished jobs are read back by the server and released here.
+    Finished jobs are read back by the server and released here. `backlog_growth_per_s` is the
+    least-squares slope of the backlog (arrived, not yet admitted) over the second half of the
+    window: near zero when the server keeps up with the arrivals.
     """
     from velvet_amd.server import Server
     rng = np.random.default_rng(0)
     lat, work, live, waits = [], 0, [], []
     phases = {"admit": [], "frame": [], "retire": []}
     backlog = []
+    trace_t, trace_b = [], []
     arrival = {}
     arrived = submitted = admitted = completed = 0
     k = 0
@@ -267,6 +353,8 @@ def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms
             st = sv.step()
             t_end = time.perf_counter() - t0
             queued = st.queued_jobs
+            trace_t.append(t_end)
+            trace_b.append(len(backlog) + queued)
             admitted += st.admitted
             completed += st.completed
             if st.admitted:
@@ -286,10 +374,13 @@ def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms
                 live.append(st.live_particles)
         wall = time.perf_counter() - t0
     pct = lambda a, q: float(np.percentile(a, q)) if a else None
+    half = [i for i, t in enumerate(trace_t) if t >= wall / 2]
+    growth = float(np.polyfit(np.asarray(trace_t)[half], np.asarray(trace_b)[half], 1)[0]) if len(half) > 2 else None
     return {"rate_rps": rate, "seconds": round(wall, 2), "steps": len(lat),
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "particle_substeps_per_s": work / wall, "arrived": arrived, "admitted": admitted,
-            "completed": completed, "backlog": len(backlog) + queued, "distinct_jobs": len(pool),
+            "completed": completed, "backlog": len(backlog) + queued, "backlog_growth_per_s": growth,
+            "distinct_jobs": len(pool),
             "admit_wait_p50_ms": pct(waits, 50), "admit_wait_p99_ms": pct(waits, 99),
             "phase_p50_ms": {k: pct(v, 50) for k, v in phases.items()},
             "phase_p99_ms": {k: pct(v, 99) for k, v in phases.items()},
@@ -317,16 +408,23 @@ def main():
                          "the cloths opt in and the tuner picks the hash-grid resolution")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-rates", default="50,100,200,400", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serve-rates", default="35,50,100,200,400", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-cap", type=int, default=0, help="live-particle cap; 0 = none")
     ap.add_argument("--serve-budget-ms", type=float, default=11.0,
                     help="admission budget on the predicted frame time")
     ap.add_argument("--serve-graph", type=int, default=0)
-    ap.add_argument("--serve-tuning", default="neighbor_order=1,hash_lds=0,streams=2", metavar="K=V,...",
+    ap.add_argument("--serve-tuning", default="neighbor_order=2,hash_lds=0,streams=2,hash_grid=1", metavar="K=V,...",
                     help="launch variants of the serving solver (the live set changes every step, so it is "
                          "not tuned online); the default is the measured best of a sweep on this job mix")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--configs", default="config1,config2,config3",
+                    help="single-cloth configurations measured beside the workload ('' = none)")
+    ap.add_argument("--config-steps", type=int, default=20)
+    ap.add_argument("--config-cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-long-rates", default="35,50,100",
+                    help="rates served for --serve-long-seconds instead of --serve-seconds")
+    ap.add_argument("--serve-long-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -338,6 +436,7 @@ def main():
     scenes, desc = workload_scenes(args.workload)
     for sc in scenes:
         sc.params.tune_hash_grid = 0 if args.reference_grid else 1
+    horizon = scenes[0].frames
     tot = counts(scenes)
     substeps = scenes[0].params.num_substeps
     iters = scenes[0].params.num_iterations
@@ -354,8 +453,12 @@ def main():
         log(f"tuning launch variants ({args.tune_frames} frames each)")
         tuning = solver.autotune(frames=args.tune_frames)
         log(f"  {tuning}")
-    log("registered; warming up")
-    for _ in range(args.warmup):
+    # the timed frames end at the workload's horizon (120 frames for the batch): the cloths have
+    # fallen onto their colliders and rest there, with the contacts and neighbour lists of the
+    # draped state
+    settle = max(0, horizon - args.warmup - args.steps)
+    log(f"registered; {settle} settling + {args.warmup} warm-up frames")
+    for _ in range(settle + args.warmup):
         solver.Simulate()
     solver.synchronize()
     log(f"timing {args.steps} frames")
@@ -369,13 +472,16 @@ def main():
     it_ms = sum(v["ms"] for v in kt.values() if v.get("kernel") == "k_iterate")
     it_n = sum(v["launches"] for v in kt.values() if v.get("kernel") == "k_iterate")
     avg_ms = it_ms / it_n if it_n else float("nan")
-    alg_bytes = S.algorithmic_bytes_per_iteration(tot)
+    alg_bytes = jacobi_bytes(scenes)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": traffic_src, "kernel": "k_iterate", "avg_launch_ms": round(avg_ms, 6),
-                "algorithmic_bytes_per_launch": alg_bytes}
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "reference_layout_bytes_per_launch": S.algorithmic_bytes_per_iteration(tot)}
+    if traffic:
+        roofline["traffic_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     solver.close()
 
     if args.copy_probe:
@@ -390,16 +496,30 @@ def main():
                  "batched_over_plain": value / pval}
         log(f"plain: {pval:.4e} particle-substeps/s -> batched/plain {value / pval:.2f}x")
 
-    serving = None
+    serving = serving_sustained = None
     if not args.no_serve:
         serving = []
         log("serving: building the job descriptions")
         pool = serve_pool(0)
+        long_rates = {float(x) for x in args.serve_long_rates.split(",") if x}
         for r in [float(x) for x in args.serve_rates.split(",") if x]:
-            log(f"serving: {r:g} req/s for {args.serve_seconds:g} s, cap {args.serve_cap} live particles")
-            serving.append(serve_run(r, args.serve_seconds, pool, args.serve_cap, bool(args.serve_graph),
+            secs = args.serve_long_seconds if r in long_rates else args.serve_seconds
+            log(f"serving: {r:g} req/s for {secs:g} s, cap {args.serve_cap} live particles")
+            serving.append(serve_run(r, secs, pool, args.serve_cap, bool(args.serve_graph),
                                      args.serve_budget_ms, tuning=parse_tuning(args.serve_tuning)))
             log(f"  {serving[-1]}")
+        # the highest swept rate whose backlog stopped growing (second-half slope under 2% of
+        # the arrival rate) over a long window
+        ok = [r["rate_rps"] for r in serving if r["rate_rps"] in long_rates and r["backlog_growth_per_s"] is not None
+              and r["backlog_growth_per_s"] <= 0.02 * r["rate_rps"]]
+        serving_sustained = max(ok) if ok else None
+
+    configs = {}
+    for name in [x for x in args.configs.split(",") if x]:
+        log(f"{name}: tuned, {args.config_steps} frames timed at the end of its horizon")
+        configs[name] = config_leg(name, args.config_steps, 2, 4 if args.tune_frames > 0 else 0,
+                                   0.0 if args.no_cpu else args.config_cpu_seconds)
+        log(f"  {configs[name]['value']:.4e} particle-substeps/s, {configs[name]['ms_per_step']:.3f} ms/frame")
 
     cpu = cpu_threads = None
     if not args.no_cpu:
@@ -424,13 +544,14 @@ def main():
             "data": "synthetic (seeded cloths)",
             "config": {"workload": f"{args.workload}: {desc}", "particles": tot["N"],
                        "stretch": tot["S"], "bend": tot["B"], "attach": tot["A"],
-                       "substeps": substeps, "iterations": iters, "tuning": tuning},
+                       "substeps": substeps, "iterations": iters, "tuning": tuning,
+                       "frames_timed": [settle + args.warmup + 1, settle + args.warmup + args.steps]},
             "plain": plain,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "frames": len(lat)},
             "kernels_ms": {k: v for k, v in kt.items()},
-            "serving": serving, "roofline": roofline, "cpu_baseline": cpu,
-            "cpu_baseline_threads": cpu_threads}
+            "serving": serving, "serving_sustained_rate_rps": serving_sustained, "roofline": roofline, "cpu_baseline": cpu,
+            "cpu_baseline_threads": cpu_threads, "configs": configs}
     print(json.dumps(line), flush=True)
 
 
