@@ -699,31 +699,6 @@ static void fill_state(const Segment& sg, Prepared& pr) {
         pr.slots[k] = make_float4(sg.slots[3 * k], sg.slots[3 * k + 1], sg.slots[3 * k + 2], 0.f);
 }
 
-// Slot evaluation order of a grid tile: a round of 256 threads takes slots j, j + 256 of each
-// type's owner range in row-major or column-major order, and a 64-lane group with no present
-// slot is skipped; pick the order with fewer groups holding a present slot (ranges as in the
-// grid pass: X rows/cols [-1, 15], R rows [-1, 16], D cols [-1, 16] around the tile origin).
-static uint32_t slot_order(const Prepared& pr, uint32_t a0, uint32_t b0) {
-    int groups[2] = {0, 0};
-    for (int type = 0; type < 3; type++) {
-        const int H = type == 1 ? 18 : 17, W = type == 2 ? 18 : 17;
-        for (int order = 0; order < 2; order++) {
-            for (int g0 = 0; g0 < H * W; g0 += 64) {
-                bool any = false;
-                for (int jj = g0; jj < std::min(g0 + 64, H * W) && !any; jj++) {
-                    const int ra = order ? jj % H - 1 : jj / W - 1;
-                    const int rb = order ? jj / H - 1 : jj % W - 1;
-                    const int64_t a = (int64_t)a0 + ra, b = (int64_t)b0 + rb;
-                    if (a < 0 || b < 0 || a >= (int64_t)pr.grows || b >= (int64_t)pr.gcols) continue;
-                    any = (pr.gflags[(size_t)a * pr.gcols + (size_t)b] >> type) & 1u;
-                }
-                groups[order] += any ? 1 : 0;
-            }
-        }
-    }
-    return groups[1] < groups[0] ? 1u : 0u;
-}
-
 // Host packing of one segment into its position-independent device layout.
 void Engine::prepare(int32_t si) {
     Segment& sg = seg(si);
@@ -834,7 +809,6 @@ void Engine::prepare(int32_t si) {
                 GridTileDev gt{};
                 gt.a0 = a0;
                 gt.b0 = b0;
-                gt.order = slot_order(pr, a0, b0);
                 std::vector<int64_t> lanes(VTX_TILE, -1);
                 for (uint32_t l = 0; l < VTX_TILE; l++) {
                     const uint32_t a = a0 + l / VTX_GT, b = b0 + l % VTX_GT;

@@ -479,29 +479,31 @@ __device__ __forceinline__ int slot_index(int ra, int rb) {
     return TYPE == 2 ? (ra + 1) * 18 + (rb + 1) : (ra + 1) * 17 + (rb + 1);
 }
 
-// Evaluate the j-th bending slot of a grid tile's range for one type, taken in row-major or
-// (colmajor) column-major order, into LDS at its row-major index. A tile cut by the cloth's
-// right edge takes column-major order, so that whole 64-lane groups of absent slots are skipped.
-template <int TYPE, bool COLMAJOR>
+// Owner cells of one slot type that can hold a slot in a tile, tile-relative: rows
+// [ra0, ra0 + h), columns [rb0, rb0 + w). The type's owner range of the tile (X: rows and columns
+// -1..15, R: rows -1..16, D: columns -1..16) clipped to the cells where the slot is geometrically
+// possible (X: a <= R-2, b <= C-2; R: also a >= 1; D: also b >= 1). h or w <= 0: none.
+template <int TYPE>
+__device__ __forceinline__ void slot_rect(int a0, int b0, int rows, int cols, int& ra0, int& rb0, int& h, int& w) {
+    constexpr int RA1 = TYPE == 1 ? 16 : 15, RB1 = TYPE == 2 ? 16 : 15;
+    ra0 = max(-1, (TYPE == 1 ? 1 : 0) - a0);
+    rb0 = max(-1, (TYPE == 2 ? 1 : 0) - b0);
+    h = min(RA1, rows - 2 - a0) - ra0 + 1;
+    w = min(RB1, cols - 2 - b0) - rb0 + 1;
+}
+
+// Evaluate the bending slot owned at tile-relative cell (ra, rb) of one type (a cell of
+// slot_rect's rectangle) into LDS at its row-major index in the type's range; act (cleared
+// before) marks the slots that contribute.
+template <int TYPE>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
-                                          float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int rows,
-                                          int cols, uint32_t jj, float xpbd) {
+                                          float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int cols,
+                                          int ra, int rb, float xpbd) {
     constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
-    constexpr int H = TYPE == 1 ? 18 : 17;                                  // owner rows of the range
-    const int ra = COLMAJOR ? (int)(jj % H) - 1 : (int)(jj / W) - 1;
-    const int rb = COLMAJOR ? (int)(jj / H) - 1 : (int)(jj % W) - 1;
     const uint32_t j = (uint32_t)((ra + 1) * W + (rb + 1));                 // row-major storage index
     const int a = a0 + ra, b = b0 + rb;
-    bool on = false;
-    if (a >= 0 && a < rows && b >= 0 && b < cols) {
-        if ((sg.flags >> (6 + TYPE)) & 1u) {
-            on = false;
-        } else if ((sg.flags >> (3 + TYPE)) & 1u) {   // X: a<=R-2,b<=C-2; R: 1<=a<=R-2,b<=C-2; D: a<=R-2,1<=b<=C-2
-            on = a + 1 < rows && b + 1 < cols && (TYPE != 1 || a >= 1) && (TYPE != 2 || b >= 1);
-        } else {
-            on = (d.gflags[sg.pbase + (uint32_t)(a * cols + b)] >> TYPE) & 1u;
-        }
-    }
+    // in the rectangle every slot of a FULL type is present; otherwise the cell's flags tell
+    bool on = ((sg.flags >> (3 + TYPE)) & 1u) || ((d.gflags[sg.pbase + (uint32_t)(a * cols + b)] >> TYPE) & 1u);
     if (on) {
         const int o = (ra + VTX_GH) * VTX_GP + (rb + VTX_GH);    // owner in staged coordinates
         // canonical vertex order per type (vtx_device.hpp)
@@ -525,26 +527,36 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
             corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
             corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
             corr[9][j] = c3.x; corr[10][j] = c3.y; corr[11][j] = c3.z;
+            act[j] = 1;
         }
     }
-    act[j] = on ? 1 : 0;
 }
 
-// One slot type: evaluate its slots (t, t + 256), then every tile vertex adds its four
-// memberships of this type in role order.
-template <int TYPE, bool COLMAJOR, int BT>
+// One slot type: the workgroup's threads take the cells of the type's rectangle densely (thread
+// t: row t / w, column t % w), so a tile cut by the cloth's edge runs no lanes on cells that
+// cannot hold a slot; then every tile vertex adds its four memberships of this type in role
+// order.
+template <int TYPE, int BT>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
                                                float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
                                                V3& delta, int& cnt) {
-    constexpr int NS = TYPE == 0 ? VTX_NSX : TYPE == 1 ? VTX_NSR : VTX_NSD;
     const uint32_t tid = threadIdx.x;
-    if (BT >= NS) {   // one slot per thread
-        if (tid < NS) eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
-    } else {          // two rounds
-        static_assert(2 * BT >= NS, "a slot type fits two rounds of the workgroup");
-        eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, tid, xpbd);
-        if (tid < NS - BT) eval_slot<TYPE, COLMAJOR>(d, sg, sp, corr, act, a0, b0, rows, cols, BT + tid, xpbd);
+    if (!((sg.flags >> (6 + TYPE)) & 1u)) {   // the type is present at all
+        int ra0, rb0, h, w;
+        slot_rect<TYPE>(a0, b0, rows, cols, ra0, rb0, h, w);
+        if (h > 0 && w > 0) {
+            const uint32_t n = (uint32_t)(h * w), m = (65536u + (uint32_t)w - 1u) / (uint32_t)w;   // t / w = t m >> 16 (t < 1024, w <= 18)
+            static_assert(2 * BT >= VTX_NSMAX, "a slot type fits two rounds of the workgroup");
+#pragma unroll
+            for (uint32_t r = 0; r < (BT >= VTX_NSMAX ? 1u : 2u); r++) {
+                const uint32_t t = tid + r * BT;
+                if (t < n) {
+                    const uint32_t q = (t * m) >> 16;
+                    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, cols, ra0 + (int)q, rb0 + (int)(t - q * (uint32_t)w), xpbd);
+                }
+            }
+        }
     }
     __syncthreads();
     if (inside) {
@@ -589,7 +601,8 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     constexpr int BT = VTX_TILE * SPLIT;
     __shared__ float4 sp[VTX_GP * VTX_GP];
     __shared__ float corr[12][VTX_NSMAX];          // plane r*3+c: component c of role r
-    __shared__ uint8_t act[VTX_NSMAX];
+    __shared__ uint32_t actw[(VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4];   // per type: slot contributes
+    uint8_t* const act = reinterpret_cast<uint8_t*>(actw);
     const GridTileDev t = tiles[tile_of(d.remap)];
     const SegDev sg = d.segs[t.seg];
     const int rows = (int)sg.rows, cols = (int)sg.cols;
@@ -599,7 +612,8 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     const uint32_t qpart = SPLIT == 1 ? 0u : tid / VTX_TILE;
     const float4* __restrict__ P = d.P[in];
 
-    // stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
+    // clear the slot marks; stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
+    for (uint32_t e = tid; e < (VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4; e += BT) actw[e] = 0u;
     for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += BT) {
         const int a = a0 - VTX_GH + (int)(e / VTX_GP), b = b0 - VTX_GH + (int)(e % VTX_GP);
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -661,15 +675,10 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
-    if (t.order == 0) {   // tile-uniform: one branch, each order compiled on its own
-        bend_type_pass<0, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<1, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<2, false, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-    } else {
-        bend_type_pass<0, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<1, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-        bend_type_pass<2, true, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-    }
+    bend_type_pass<0, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    bend_type_pass<1, BT>(d, sg, sp, corr, act + VTX_NSX, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    bend_type_pass<2, BT>(d, sg, sp, corr, act + VTX_NSX + VTX_NSR, a0, b0, rows, cols, xpbd_bend, inside, ty, tx,
+                          delta, cnt);
     if (!inside) return;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);

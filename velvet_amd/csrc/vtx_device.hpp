@@ -98,8 +98,7 @@ struct alignas(16) SortTileDev {
 };
 
 struct alignas(16) GridTileDev {
-    uint32_t seg, a0, b0, order;           // segment, first row / column of the tile, slot
-                                           // evaluation order (0 row-major, 1 column-major)
+    uint32_t seg, a0, b0, pad0;            // segment, first row / column of the tile
     uint32_t at_off, at_k, pad1, pad2;     // attach ELL of the tile's lanes
 };
 
