@@ -303,7 +303,7 @@ def serve_pool(seed, count=64):
 
 
 def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms=0.0, ahead=4,
-              reserve_particles=6_000_000, tuning=None):
+              reserve_particles=6_000_000, tuning=None, pipelined=True):
     """Poisson arrivals at `rate` req/s for `seconds` of wall time against the serving loop.
 
     Request k is job description pool[k % len(pool)]. Arrived requests wait in a backlog; the
@@ -327,7 +327,7 @@ def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms
     k = 0
     next_t = rng.exponential(1.0 / rate)
     with Server(max_live_particles=max_live_particles, use_graph=use_graph,
-                step_budget_ms=step_budget_ms, reserve_particles=reserve_particles) as sv:
+                step_budget_ms=step_budget_ms, reserve_particles=reserve_particles, pipelined=pipelined) as sv:
         if tuning:
             sv.solver.set_tuning(**tuning)
         tun = sv.solver.tuning()
@@ -386,7 +386,7 @@ def serve_run(rate, seconds, pool, max_live_particles, use_graph, step_budget_ms
             "phase_p99_ms": {k: pct(v, 99) for k, v in phases.items()},
             "mean_live_particles": float(np.mean(live)) if live else 0.0,
             "max_live_particles_cap": max_live_particles, "step_budget_ms": step_budget_ms,
-            "reserve_particles": reserve_particles, "tuning": tun}
+            "reserve_particles": reserve_particles, "pipelined": pipelined, "tuning": tun}
 
 
 def main():
@@ -414,6 +414,9 @@ def main():
     ap.add_argument("--serve-budget-ms", type=float, default=11.0,
                     help="admission budget on the predicted frame time")
     ap.add_argument("--serve-graph", type=int, default=0)
+    ap.add_argument("--serve-pipelined", type=int, default=1,
+                    help="1: each serving step returns once its frame is launched, so request handling "
+                         "overlaps the frame; 0: each step waits for its frame")
     ap.add_argument("--serve-tuning", default="neighbor_order=2,hash_lds=0,streams=2,hash_grid=1", metavar="K=V,...",
                     help="launch variants of the serving solver (the live set changes every step, so it is "
                          "not tuned online); the default is the measured best of a sweep on this job mix")
@@ -506,7 +509,8 @@ def main():
             secs = args.serve_long_seconds if r in long_rates else args.serve_seconds
             log(f"serving: {r:g} req/s for {secs:g} s, cap {args.serve_cap} live particles")
             serving.append(serve_run(r, secs, pool, args.serve_cap, bool(args.serve_graph),
-                                     args.serve_budget_ms, tuning=parse_tuning(args.serve_tuning)))
+                                     args.serve_budget_ms, tuning=parse_tuning(args.serve_tuning),
+                                     pipelined=bool(args.serve_pipelined)))
             log(f"  {serving[-1]}")
         # the highest swept rate whose backlog stopped growing (second-half slope under 2% of
         # the arrival rate) over a long window

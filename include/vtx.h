@@ -82,7 +82,7 @@ typedef struct vtx_solver vtx_solver;
 
 /* Layout version of the structs above and below; vtx_abi_version() returns the library's, so a
  * binding can check at load time that it was written against the same layouts. */
-#define VTX_ABI_VERSION 4
+#define VTX_ABI_VERSION 5
 
 void vtx_params_default(vtx_params* p);
 const char* vtx_last_error(void);
@@ -351,16 +351,25 @@ typedef struct vtx_server_config {
     uint64_t reserve_particles;    /* size the device pools at creation for this many live
                                       particles of grid cloths, so admission never waits for a
                                       pool to grow; 0 = grow on demand */
+    int32_t pipelined;             /* 1: vtx_server_step returns once the frame is launched; the
+                                      next step (or vtx_server_drain) waits for it and retires
+                                      finished jobs, so the caller's work between steps (new
+                                      jobs, polling) overlaps the frame on the GPU. 0: each step
+                                      waits for its own frame */
 } vtx_server_config;
 
 typedef struct vtx_server_stats {
     uint32_t live_jobs, queued_jobs, admitted, completed, failed, pad;
     uint64_t live_particles;
-    uint64_t particle_substeps;    /* work done by this step */
-    double step_ms;                /* wall time of this step (admission + frame + retirement) */
-    double admit_ms;               /* of which: uploading admitted jobs and rebuilding the tables */
-    double frame_ms;               /* of which: the frame of every live job */
-    double retire_ms;              /* of which: reading back and removing finished jobs */
+    uint64_t particle_substeps;    /* work of the frame completed by this step */
+    double step_ms;                /* wall time of this step (admission + frame + retirement);
+                                      pipelined: from the previous step's frame launch to this
+                                      step's (or to this step's end if it launches none), the
+                                      period at which live jobs advance */
+    double admit_ms;               /* uploading admitted jobs and rebuilding the tables */
+    double frame_ms;               /* GPU time of the frame completed by this step (pipelined:
+                                      the previous step's frame) */
+    double retire_ms;              /* reading back and removing finished jobs */
 } vtx_server_stats;
 
 typedef struct vtx_server vtx_server;
@@ -378,6 +387,9 @@ int vtx_server_release_job(vtx_server* sv, int32_t job);
  * again behind the waiting jobs; it resumes from the frame it reached with the same result as
  * an uninterrupted run. */
 int vtx_server_preempt(vtx_server* sv, int32_t job);
+/* Pipelined servers: wait for the frame in flight and retire the jobs it finished (their
+ * statistics go to the next step's). A no-op when no frame is in flight. */
+int vtx_server_drain(vtx_server* sv);
 
 #ifdef __cplusplus
 }

@@ -43,13 +43,15 @@ def alone(scene, frames):
         return s.positions()
 
 
-@pytest.mark.parametrize("use_graph,reserve", [(False, 0), (True, 0), (False, 200_000)])
-def test_mix_equals_alone(use_graph, reserve):
+@pytest.mark.parametrize("use_graph,reserve,pipelined", [(False, 0, False), (True, 0, False), (False, 200_000, False),
+                                                        (False, 0, True), (True, 200_000, True)])
+def test_mix_equals_alone(use_graph, reserve, pipelined):
     """Jobs that arrive while others run are uploaded during a frame (staged) and join at the
-    next step; every result equals the job's solo run, with or without pre-sized pools."""
-    say(f"serving mix vs alone graph={use_graph} reserve={reserve}")
+    next step; every result equals the job's solo run, with or without pre-sized pools, and with
+    steps that return while their frame runs (pipelined)."""
+    say(f"serving mix vs alone graph={use_graph} reserve={reserve} pipelined={pipelined}")
     js = jobs(5)
-    with Server(use_graph=use_graph, reserve_particles=reserve) as sv:
+    with Server(use_graph=use_graph, reserve_particles=reserve, pipelined=pipelined) as sv:
         ids = []
         # staggered arrivals: two now, one after 2 steps, two after 5
         arrivals = {0: [0, 1], 2: [2], 5: [3, 4]}
@@ -129,11 +131,13 @@ def test_step_time_budget_admission():
         assert all(sv.status(j)[0] == DONE for j in ids)
 
 
-def test_preempt_resume_equals_alone():
-    """A running job taken off the device and re-admitted later ends bit-identical to its solo run."""
-    say("serving preemption")
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_preempt_resume_equals_alone(pipelined):
+    """A running job taken off the device and re-admitted later ends bit-identical to its solo run
+    (pipelined: preemption first completes the frame in flight)."""
+    say(f"serving preemption pipelined={pipelined}")
     js = jobs(3)
-    with Server() as sv:
+    with Server(pipelined=pipelined) as sv:
         ids = [sv.submit(sc, f) for sc, f in js]
         sv.step()
         sv.step()
@@ -211,3 +215,25 @@ def _has_segment(solver, seg):
         return True
     except Exception:
         return False
+
+
+def test_pipelined_steps_lag_by_one_frame_and_drain():
+    """A pipelined step returns with its frame still running: the frame is counted (frames done,
+    completed jobs, frame time) by the next step or by drain(), and the results match a
+    synchronous server's bit for bit."""
+    say("serving pipelined step accounting")
+    js = jobs(2)
+    with Server(pipelined=True) as sv:
+        a = sv.submit(js[0][0], 3)
+        st = sv.step()                       # frame 1 launched
+        assert st.frame_ms == 0.0 and st.particle_substeps == 0 and sv.status(a) == (RUNNING, 0)
+        st = sv.step()                       # frame 1 counted, frame 2 launched
+        assert st.frame_ms > 0.0 and st.particle_substeps > 0 and sv.status(a) == (RUNNING, 1)
+        st = sv.step()                       # frame 2 counted, frame 3 (the last) launched
+        assert sv.status(a) == (RUNNING, 2)
+        sv.drain()                           # frame 3 counted, the job retired
+        assert sv.status(a) == (DONE, 3)
+        st = sv.step()                       # the drained frame's figures arrive here
+        assert st.completed == 1 and st.frame_ms > 0.0 and st.live_jobs == 0
+        res = sv.result(a)
+    assert np.array_equal(res, alone(js[0][0], 3))

@@ -17,7 +17,7 @@ i32, u32, f32 = C.c_int32, C.c_uint32, C.c_float
 P = C.POINTER
 
 
-ABI_VERSION = 4   # VTX_ABI_VERSION of include/vtx.h for the structures below
+ABI_VERSION = 5   # VTX_ABI_VERSION of include/vtx.h for the structures below
 
 
 class VtxParams(C.Structure):
@@ -38,7 +38,8 @@ class VtxCollider(C.Structure):
 class VtxServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("hbm_reserve_bytes", C.c_uint64),
                 ("max_live_particles", C.c_uint64), ("max_live_jobs", u32), ("use_graph", i32),
-                ("check_every", i32), ("step_budget_ms", f32), ("reserve_particles", C.c_uint64)]
+                ("check_every", i32), ("step_budget_ms", f32), ("reserve_particles", C.c_uint64),
+                ("pipelined", i32)]
 
 
 class VtxTuning(C.Structure):
@@ -117,6 +118,7 @@ _SIGS = {
     "vtx_read_timing": (i32, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "vtx_server_create": (i32, [P(VtxServerConfig), P(C.c_void_p)]),
     "vtx_server_preempt": (i32, [C.c_void_p, i32]),
+    "vtx_server_drain": (i32, [C.c_void_p]),
     "vtx_server_destroy": (i32, [C.c_void_p]),
     "vtx_server_solver": (i32, [C.c_void_p, P(C.c_void_p)]),
     "vtx_server_new_job": (i32, [C.c_void_p, P(VtxParams), i32, P(i32), P(i32)]),

@@ -538,4 +538,9 @@ int vtx_server_release_job(vtx_server* sv, int32_t job) {
     API_BEGIN sv->server.release(job); API_END
 }
 
+int vtx_server_drain(vtx_server* sv) {
+    NEED(sv);
+    API_BEGIN sv->server.drain(); API_END
+}
+
 }  // extern "C"

@@ -15,6 +15,7 @@ Server::Server(Engine& e, const vtx_server_config& cfg) : e_(e), cfg_(cfg) {
 }
 
 Server::~Server() {
+    if (in_flight_) (void)hipStreamSynchronize(e_.stream());
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
 }
@@ -107,49 +108,28 @@ uint32_t Server::admit(bool stage, uint32_t before) {
     return n;
 }
 
-void Server::step(float frame_dt, vtx_server_stats* out) {
+// Wait for the frame in flight, count it for its jobs and the cost model, then retire the jobs
+// that reached their horizon (read back and removed) or diverged; the figures go to carry_.
+void Server::complete() {
     using clk = std::chrono::steady_clock;
-    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
-    const auto t0 = clk::now();
-    vtx_server_stats st{};
-    for (int32_t id : staged_) {                 // staged last step: uploaded, now live
-        e_.set_active(get(id).seg, true);
-        live_.push_back(id);
-    }
-    staged_.clear();
-    if (live_.empty()) st.admitted += admit(false, 0);
+    if (!in_flight_) return;
+    e_.synchronize();
+    in_flight_ = false;
+    float frame_ms = 0.f;
+    VTX_HIP(hipEventElapsedTime(&frame_ms, ev0_, ev1_));
+    carry_.frame_ms += frame_ms;
     uint64_t work = 0;
-    if (!live_.empty()) {
-        const auto c0 = clk::now();
-        e_.commit();                             // uploads of jobs activated now, launch tables
-        st.admit_ms = ms_since(c0);
-        if (!ev0_) {
-            VTX_HIP(hipEventCreate(&ev0_));
-            VTX_HIP(hipEventCreate(&ev1_));
-        }
-        const hipStream_t es = e_.stream();
-        VTX_HIP(hipEventRecord(ev0_, es));
-        e_.simulate(frame_dt);
-        e_.join();                               // the frame's end is the end of every stream group
-        VTX_HIP(hipEventRecord(ev1_, es));
-        const auto s0 = clk::now();
-        st.admitted += admit(true, st.admitted);   // overlaps the frame
-        st.admit_ms += ms_since(s0);
-        e_.synchronize();
-        float frame_ms = 0.f;
-        VTX_HIP(hipEventElapsedTime(&frame_ms, ev0_, ev1_));
-        st.frame_ms = frame_ms;
-        for (int32_t id : live_) {
-            Job& j = get(id);
-            j.done++;
-            work += j.work;
-        }
-        if (work) {
-            const double r = frame_ms / (double)work;
-            ms_per_work_ = ms_per_work_ == 0.0 ? r : 0.8 * ms_per_work_ + 0.2 * r;
-        }
+    for (int32_t id : flight_) {
+        Job& j = get(id);
+        j.done++;
+        work += j.work;
     }
-    steps_++;
+    flight_.clear();
+    carry_.particle_substeps += work;
+    if (work) {
+        const double r = frame_ms / (double)work;
+        ms_per_work_ = ms_per_work_ == 0.0 ? r : 0.8 * ms_per_work_ + 0.2 * r;
+    }
     const auto r0 = clk::now();
     std::vector<int32_t> bad;
     if (cfg_.check_every > 0 && steps_ % (uint64_t)cfg_.check_every == 0 && !live_.empty())
@@ -170,18 +150,79 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
             resident_ -= j.bytes;
             live_particles_ -= j.n;
             live_work_ -= j.work;
-            (failed ? st.failed : st.completed)++;
+            (failed ? carry_.failed : carry_.completed)++;
         } else {
             keep.push_back(id);
         }
     }
     live_.swap(keep);
-    st.retire_ms = ms_since(r0);
+    carry_.retire_ms += std::chrono::duration<double, std::milli>(clk::now() - r0).count();
+}
+
+void Server::drain() { complete(); }
+
+// One step: (pipelined: complete the previous step's frame first) staged jobs join, queued jobs
+// are admitted, the frame of every live job is launched, jobs admitted while it runs are
+// uploaded on the copy stream; without pipelining the step then waits for its frame and
+// retires, with pipelining it returns and the next step does.
+void Server::step(float frame_dt, vtx_server_stats* out) {
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    const auto t0 = clk::now();
+    complete();                                  // pipelined: the previous step's frame
+    vtx_server_stats st = carry_;
+    carry_ = vtx_server_stats{};
+    for (int32_t id : staged_) {                 // staged last step: uploaded, now live
+        e_.set_active(get(id).seg, true);
+        live_.push_back(id);
+    }
+    staged_.clear();
+    if (live_.empty()) st.admitted += admit(false, 0);
+    bool launched = false;
+    if (!live_.empty()) {
+        const auto c0 = clk::now();
+        e_.commit();                             // uploads of jobs activated now, launch tables
+        st.admit_ms += ms_since(c0);
+        if (!ev0_) {
+            VTX_HIP(hipEventCreate(&ev0_));
+            VTX_HIP(hipEventCreate(&ev1_));
+        }
+        const hipStream_t es = e_.stream();
+        const auto launch = clk::now();          // pipelined step_ms: launch to launch
+        if (cfg_.pipelined && last_launch_ != clk::time_point{})
+            st.step_ms = std::chrono::duration<double, std::milli>(launch - last_launch_).count();
+        last_launch_ = launch;
+        VTX_HIP(hipEventRecord(ev0_, es));
+        e_.simulate(frame_dt);
+        e_.join();                               // the frame's end is the end of every stream group
+        VTX_HIP(hipEventRecord(ev1_, es));
+        in_flight_ = true;
+        flight_ = live_;
+        launched = true;
+        const auto s0 = clk::now();
+        st.admitted += admit(true, st.admitted);   // overlaps the frame
+        st.admit_ms += ms_since(s0);
+    }
+    steps_++;
+    if (!cfg_.pipelined) {
+        complete();
+        st.frame_ms += carry_.frame_ms;
+        st.particle_substeps += carry_.particle_substeps;
+        st.completed += carry_.completed;
+        st.failed += carry_.failed;
+        st.retire_ms += carry_.retire_ms;
+        carry_ = vtx_server_stats{};
+    }
     st.live_jobs = (uint32_t)live_.size();
     st.queued_jobs = (uint32_t)queue_.size();
     st.live_particles = live_particles_;
-    st.particle_substeps = work;
-    st.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (cfg_.pipelined && !launched && last_launch_ != clk::time_point{}) {
+        // the last frame's period ends here; the next launch starts a new run of periods
+        st.step_ms = std::chrono::duration<double, std::milli>(clk::now() - last_launch_).count();
+        last_launch_ = clk::time_point{};
+    } else if (!(cfg_.pipelined && launched && st.step_ms > 0.0)) {   // first launch, or synchronous
+        st.step_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
     if (out) *out = st;
 }
 
@@ -189,6 +230,7 @@ void Server::step(float frame_dt, vtx_server_stats* out) {
 // queues again behind the jobs already waiting; when re-admitted it continues from the frame it
 // reached, with the same result as an uninterrupted run.
 void Server::preempt(int32_t id) {
+    complete();
     Job& j = get(id);
     VTX_REQUIRE(j.status == VTX_JOB_RUNNING, VTX_ESTATE, "only a running job can be preempted");
     e_.park(j.seg);

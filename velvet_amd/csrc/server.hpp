@@ -34,11 +34,13 @@ public:
     const Job& job(int32_t id) const;
     void release(int32_t id);
     void preempt(int32_t id);
+    void drain();                // pipelined: complete the frame in flight
 
 private:
     Job& get(int32_t id);
     bool fits(const Job& j, uint32_t admitted_now) const;
     uint32_t admit(bool stage, uint32_t before);   // before: jobs admitted earlier in this step
+    void complete();             // wait for the frame in flight, count it, retire finished jobs
 
     Engine& e_;
     vtx_server_config cfg_;
@@ -54,6 +56,10 @@ private:
     uint64_t live_work_ = 0;     // particle-substeps per frame of the live jobs
     double ms_per_work_ = 0.0;   // measured frame time per particle-substep (EMA); 0 = not yet
     uint64_t steps_ = 0;
+    bool in_flight_ = false;     // a launched frame not yet completed
+    std::vector<int32_t> flight_;   // the jobs of that frame
+    vtx_server_stats carry_{};   // completed-frame figures for the next step's statistics
+    std::chrono::steady_clock::time_point last_launch_{};
 };
 
 }  // namespace vtx

@@ -23,11 +23,13 @@ STATUS = {0: "pending", 1: "queued", 2: "running", 3: "done", 4: "failed", 5: "r
 class Server:
     def __init__(self, hbm_budget_bytes: int = 0, hbm_reserve_bytes: int = 4 << 30,
                  max_live_particles: int = 0, max_live_jobs: int = 0, use_graph: bool = False,
-                 check_every: int = 0, step_budget_ms: float = 0.0, reserve_particles: int = 0):
+                 check_every: int = 0, step_budget_ms: float = 0.0, reserve_particles: int = 0,
+                 pipelined: bool = False):
         cfg = VtxServerConfig(hbm_budget_bytes=hbm_budget_bytes, hbm_reserve_bytes=hbm_reserve_bytes,
                               max_live_particles=max_live_particles, max_live_jobs=max_live_jobs,
                               use_graph=int(bool(use_graph)), check_every=check_every,
-                              step_budget_ms=step_budget_ms, reserve_particles=reserve_particles)
+                              step_budget_ms=step_budget_ms, reserve_particles=reserve_particles,
+                              pipelined=int(bool(pipelined)))
         h = C.c_void_p()
         check(lib().vtx_server_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -67,6 +69,10 @@ class Server:
         st = VtxServerStats()
         check(lib().vtx_server_step(self.h, float(S.FRAME_DT if frame_dt is None else frame_dt), C.byref(st)))
         return st
+
+    def drain(self):
+        """Pipelined servers: wait for the frame in flight and retire what it finished."""
+        check(lib().vtx_server_drain(self.h))
 
     def status(self, job: int):
         s, d = C.c_int32(), C.c_int32()
