@@ -378,6 +378,9 @@ int vtx_server_create(const vtx_server_config* cfg, vtx_server** out);
 int vtx_server_destroy(vtx_server* sv);
 int vtx_server_solver(vtx_server* sv, vtx_solver** out);
 int vtx_server_new_job(vtx_server* sv, const vtx_params* params, int32_t frames, int32_t* job, int32_t* seg);
+/* Queue a job. Its host packing runs on the server's worker thread while the caller goes on
+ * (registering more jobs, stepping); the job's first admission attempt or status query waits for
+ * that packing. A submitted job's segment must not be changed any more. */
 int vtx_server_submit(vtx_server* sv, int32_t job);
 int vtx_server_step(vtx_server* sv, float frame_dt, vtx_server_stats* out);
 int vtx_server_job_status(vtx_server* sv, int32_t job, int32_t* status, int32_t* frames_done);

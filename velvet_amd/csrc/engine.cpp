@@ -622,12 +622,11 @@ void Engine::add_attach(int32_t s, const int32_t* pid, const int32_t* slot, cons
         VTX_REQUIRE(slot[c] >= 0 && slot[c] < L, VTX_EINVAL, "attach slot out of range");
     }
     touch(s);
-    for (uint32_t c = 0; c < count; c++) {
+    for (uint32_t c = 0; c < count; c++)
         if (dist[c] == 0.0f) sg.w[(size_t)pid[c]] = 0.0f;
-        sg.at_pid.push_back(pid[c]);
-        sg.at_slot.push_back(slot[c]);
-        sg.at_dist.push_back(dist[c]);
-    }
+    sg.at_pid.insert(sg.at_pid.end(), pid, pid + count);
+    sg.at_slot.insert(sg.at_slot.end(), slot, slot + count);
+    sg.at_dist.insert(sg.at_dist.end(), dist, dist + count);
 }
 
 void Engine::add_bend(int32_t s, const uint32_t* idx, const float* angle, uint32_t count) {
@@ -700,8 +699,9 @@ static void fill_state(const Segment& sg, Prepared& pr) {
 }
 
 // Host packing of one segment into its position-independent device layout.
-void Engine::prepare(int32_t si) {
-    Segment& sg = seg(si);
+void Engine::prepare(int32_t si) { prepare_segment(seg(si)); }
+
+void Engine::prepare_segment(Segment& sg) {
     if (sg.prepared) return;
     Prepared pr;
     const uint32_t n = sg.n();
@@ -735,16 +735,21 @@ void Engine::prepare(int32_t si) {
         pr.brest4 = std::move(G.brest4);
     }
 
-    // incidence lists (CSR over local particles, registration order)
+    // incidence lists (CSR over local particles, registration order); the stencil path reads
+    // neither stretch / bend lists nor triangle lists (its normals come from the grid), so a grid
+    // segment builds only the attachment lists
     std::vector<uint32_t> s_st(n + 1, 0), s_at(n + 1, 0), s_bd(n + 1, 0), s_tr(n + 1, 0);
-    const size_t S = sg.st_rest.size(), A = sg.at_pid.size(), B = sg.bd_rest.size(), T = sg.tris.size() / 3;
-    for (size_t c = 0; c < S; c++) {
-        s_st[(size_t)sg.st[2 * c] + 1]++;
-        s_st[(size_t)sg.st[2 * c + 1] + 1]++;
+    const size_t S = sg.st_rest.size(), A = sg.at_pid.size(), B = sg.bd_rest.size();
+    const size_t T = pr.grid ? 0 : sg.tris.size() / 3;
+    if (!pr.grid) {
+        for (size_t c = 0; c < S; c++) {
+            s_st[(size_t)sg.st[2 * c] + 1]++;
+            s_st[(size_t)sg.st[2 * c + 1] + 1]++;
+        }
+        for (size_t c = 0; c < B; c++)
+            for (int r = 0; r < 4; r++) s_bd[(size_t)sg.bd[4 * c + r] + 1]++;
     }
     for (size_t c = 0; c < A; c++) s_at[(size_t)sg.at_pid[c] + 1]++;
-    for (size_t c = 0; c < B; c++)
-        for (int r = 0; r < 4; r++) s_bd[(size_t)sg.bd[4 * c + r] + 1]++;
     for (size_t c = 0; c < T; c++)
         for (int r = 0; r < 3; r++) s_tr[(size_t)sg.tris[3 * c + r] + 1]++;
     for (uint32_t i = 0; i < n; i++) {
@@ -800,7 +805,7 @@ void Engine::prepare(int32_t si) {
             pack_tile(pr.eat, s_at, l_at, first, t.count, make_uint2(VTX_INVALID, 0u), &t.at_off, &t.at_k);
             pack_tile(pr.ebd, s_bd, l_bd, first, t.count, VTX_INVALID, &t.bd_off, &t.bd_k);
         }
-        pack_tile(pr.etr, s_tr, l_tr, first, t.count, VTX_INVALID, &t.tr_off, &t.tr_k);
+        if (!pr.grid) pack_tile(pr.etr, s_tr, l_tr, first, t.count, VTX_INVALID, &t.tr_off, &t.tr_k);
         pr.tiles.push_back(t);
     }
     if (pr.grid) {
@@ -845,6 +850,8 @@ static size_t bytes_of(const Ranges& r) {
            (r.n_st + r.n_at) * sizeof(uint2) + (r.n_bd + r.n_tr) * 4 + r.n_brec * (sizeof(uint4) + 4) +
            r.n_trec * sizeof(uint4) + r.n_slot * sizeof(float4) + r.n_col * sizeof(ColliderDev) + r.n_lra * 4;
 }
+
+size_t Engine::bytes_estimate(const Segment& sg) { return bytes_of(needs_of(sg)); }
 
 size_t Engine::device_bytes_estimate(int32_t s) {
     prepare(s);

@@ -194,6 +194,11 @@ public:
     void set_colliders(int32_t s, const vtx_collider* cols, uint32_t n);
 
     void prepare(int32_t s);                 // host packing ahead of upload (optional)
+    // The same packing of one segment object; touches nothing else of the engine (it reads only
+    // the stencil-path switch), so another thread may run it on a segment that is neither
+    // active nor being registered while this engine runs frames.
+    void prepare_segment(Segment& sg);
+    static size_t bytes_estimate(const Segment& sg);   // device bytes of a prepared segment
     size_t device_bytes_estimate(int32_t s);
     void simulate(float frame_dt);
     void synchronize();

@@ -12,9 +12,16 @@ Server::Server(Engine& e, const vtx_server_config& cfg) : e_(e), cfg_(cfg) {
     }
     e_.set_use_graph(cfg_.use_graph != 0);
     e_.reserve(cfg_.reserve_particles);
+    worker_ = std::thread([this] { worker_loop(); });
 }
 
 Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    work_cv_.notify_all();
+    if (worker_.joinable()) worker_.join();
     if (in_flight_) (void)hipStreamSynchronize(e_.stream());
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -26,10 +33,64 @@ Job& Server::get(int32_t id) {
     return it->second;
 }
 
-const Job& Server::job(int32_t id) const {
-    auto it = jobs_.find(id);
-    VTX_REQUIRE(it != jobs_.end(), VTX_EINVAL, "no such job");
-    return it->second;
+const Job& Server::job(int32_t id) {
+    Job& j = get(id);
+    settle(j);
+    return j;
+}
+
+// Submitted jobs are packed here, in submission order, while the caller's thread registers
+// further jobs or runs frames. The engine is not otherwise touched from this thread: a queued
+// job's segment is inactive, so frames, commits and table rebuilds skip it, and its packing
+// reads and writes that segment object alone.
+void Server::worker_loop() {
+    for (;;) {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            work_cv_.wait(lk, [this] { return stop_ || !to_pack_.empty(); });
+            if (to_pack_.empty()) return;        // stopping, nothing left
+            j = to_pack_.front();
+            to_pack_.pop_front();
+        }
+        size_t bytes = 0;
+        uint32_t n = 0;
+        uint64_t work = 0;
+        bool reject = false;
+        try {
+            e_.prepare_segment(*j->sp);
+            bytes = Engine::bytes_estimate(*j->sp);
+            n = j->sp->n();
+            work = (uint64_t)n * (uint64_t)j->sp->prm.num_substeps;
+            reject = bytes > limit_ || (cfg_.max_live_particles && n > cfg_.max_live_particles);
+        } catch (const std::exception&) {
+            reject = true;                       // e.g. beyond a capacity limit of the engine
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            j->bytes = bytes;
+            j->n = n;
+            j->work = work;
+            j->reject = reject;
+            j->prepared = true;
+        }
+        done_cv_.notify_all();
+    }
+}
+
+void Server::settle(Job& j) {
+    if (j.status != VTX_JOB_QUEUED || j.sp == nullptr) return;
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&j] { return j.prepared; });
+    }
+    j.sp = nullptr;                              // packed: the job no longer needs the worker
+    if (j.reject) {                              // could never be admitted
+        j.status = VTX_JOB_REJECTED;
+        queue_.erase(std::find(queue_.begin(), queue_.end(), j.id));
+        e_.remove_segment(j.seg);
+        j.seg = -1;
+    }
 }
 
 // A job is a pending (inactive) segment of the server's solver; the caller registers its
@@ -45,20 +106,20 @@ int32_t Server::new_job(const vtx_params& p, int32_t frames, int32_t* seg) {
     return j.id;
 }
 
+// The job is queued at once; its host packing runs on the worker thread, and the first look
+// at the job (admission, a status query) waits for that packing to finish.
 void Server::submit(int32_t id) {
     Job& j = get(id);
     VTX_REQUIRE(j.status == VTX_JOB_PENDING, VTX_ESTATE, "job was already submitted");
-    e_.prepare(j.seg);                         // host-side packing happens at submission
-    j.bytes = e_.device_bytes_estimate(j.seg);
-    j.n = e_.seg(j.seg).n();
-    j.work = (uint64_t)j.n * (uint64_t)e_.seg(j.seg).prm.num_substeps;
-    if (j.bytes > limit_ || (cfg_.max_live_particles && j.n > cfg_.max_live_particles)) {
-        j.status = VTX_JOB_REJECTED;           // could never be admitted
-        e_.remove_segment(j.seg);
-        return;
-    }
+    j.sp = &e_.seg(j.seg);
     j.status = VTX_JOB_QUEUED;
     queue_.push_back(id);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        j.prepared = false;
+        to_pack_.push_back(&j);
+    }
+    work_cv_.notify_one();
 }
 
 // Step-time admission: the frame time of the live set is predicted as (particle-substeps) x
@@ -84,6 +145,8 @@ uint32_t Server::admit(bool stage, uint32_t before) {
     uint32_t n = 0;
     while (!queue_.empty()) {
         Job& j = get(queue_.front());
+        settle(j);
+        if (j.status == VTX_JOB_REJECTED) continue;   // settle took it out of the queue
         if (!fits(j, before + n)) break;
         if (stage) {
             // a frame is in flight: an upload that fails (e.g. device memory) leaves the job
@@ -246,6 +309,7 @@ void Server::preempt(int32_t id) {
 
 void Server::release(int32_t id) {
     Job& j = get(id);
+    settle(j);
     VTX_REQUIRE(j.status != VTX_JOB_RUNNING && j.status != VTX_JOB_QUEUED, VTX_ESTATE,
                 "job is still queued or running");
     if (j.seg >= 0) e_.remove_segment(j.seg);

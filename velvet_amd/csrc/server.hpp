@@ -4,8 +4,12 @@
 #pragma once
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -20,6 +24,11 @@ struct Job {
     uint32_t n = 0;
     uint64_t work = 0;           // particle-substeps per frame
     std::vector<float> result;   // final positions (xyz) once done
+    // host packing of a submitted job runs on the server's worker thread; these are written
+    // there under the server's mutex
+    Segment* sp = nullptr;       // the job's segment (stable while the job is queued)
+    bool prepared = false;       // packing finished (bytes, n, work valid)
+    bool reject = false;         // could never be admitted, or packing failed
 };
 
 class Server {
@@ -31,7 +40,7 @@ public:
     int32_t new_job(const vtx_params& p, int32_t frames, int32_t* seg);
     void submit(int32_t job);
     void step(float frame_dt, vtx_server_stats* out);
-    const Job& job(int32_t id) const;
+    const Job& job(int32_t id);  // waits for the job's packing if it is still running
     void release(int32_t id);
     void preempt(int32_t id);
     void drain();                // pipelined: complete the frame in flight
@@ -41,6 +50,8 @@ private:
     bool fits(const Job& j, uint32_t admitted_now) const;
     uint32_t admit(bool stage, uint32_t before);   // before: jobs admitted earlier in this step
     void complete();             // wait for the frame in flight, count it, retire finished jobs
+    void worker_loop();          // packs submitted jobs (Engine::prepare_segment) off the caller's thread
+    void settle(Job& j);         // wait for a submitted job's packing; reject it if it can never run
 
     Engine& e_;
     vtx_server_config cfg_;
@@ -60,6 +71,11 @@ private:
     std::vector<int32_t> flight_;   // the jobs of that frame
     vtx_server_stats carry_{};   // completed-frame figures for the next step's statistics
     std::chrono::steady_clock::time_point last_launch_{};
+    std::mutex mu_;              // guards the packing queue and the jobs' packing results
+    std::condition_variable work_cv_, done_cv_;
+    std::deque<Job*> to_pack_;
+    bool stop_ = false;
+    std::thread worker_;
 };
 
 }  // namespace vtx

@@ -347,15 +347,17 @@ class ClothSolver:
             self._seg0_used = True
         self.set_params(scene.params, seg)
         slot_base = 0
+        def shifted(a, k):    # index arrays offset by k (no copy for k = 0, the usual single cloth)
+            return a if k == 0 else a.astype(np.int64) + k
+
         for c in scene.cloths:
             off = self.AddCloth(c.x, c.tris, c.diameter, seg=seg)
-            self.AddStretchN(c.stretch.astype(np.int64) + off, c.stretch_rest, seg=seg)
+            self.AddStretchN(shifted(c.stretch, off), c.stretch_rest, seg=seg)
             for s in range(c.slots.shape[0]):
                 self.AddAttachSlot(c.slots[s], seg=seg)
             if c.attach_pid.size:
-                self.AddAttachN(c.attach_pid.astype(np.int64) + off,
-                                c.attach_slot.astype(np.int64) + slot_base, c.attach_dist, seg=seg)
-            self.AddBendN(c.bend.astype(np.int64) + off, c.bend_rest, seg=seg)
+                self.AddAttachN(shifted(c.attach_pid, off), shifted(c.attach_slot, slot_base), c.attach_dist, seg=seg)
+            self.AddBendN(shifted(c.bend, off), c.bend_rest, seg=seg)
             slot_base += c.slots.shape[0]
         # AddCloth set diameter / max speed to the values scene.params already holds
         self.set_params(scene.params, seg)
