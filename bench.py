@@ -408,7 +408,7 @@ def main():
                          "the cloths opt in and the tuner picks the hash-grid resolution")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-rates", default="35,50,100,200,400", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serve-rates", default="50,60,100,200,400", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-cap", type=int, default=0, help="live-particle cap; 0 = none")
     ap.add_argument("--serve-budget-ms", type=float, default=11.0,
@@ -425,7 +425,7 @@ def main():
                     help="single-cloth configurations measured beside the workload ('' = none)")
     ap.add_argument("--config-steps", type=int, default=20)
     ap.add_argument("--config-cpu-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-long-rates", default="35,50,100",
+    ap.add_argument("--serve-long-rates", default="50,60,100",
                     help="rates served for --serve-long-seconds instead of --serve-seconds")
     ap.add_argument("--serve-long-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
