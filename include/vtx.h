@@ -337,7 +337,8 @@ int vtx_read_timing(vtx_solver* s, char* buf, size_t len);
 #define VTX_JOB_RUNNING 2
 #define VTX_JOB_DONE 3
 #define VTX_JOB_FAILED 4     /* evicted: non-finite state detected */
-#define VTX_JOB_REJECTED 5   /* larger than the whole budget */
+#define VTX_JOB_REJECTED 5   /* larger than the whole budget, or beyond a capacity limit of the
+                                solver (seen at the job's first admission attempt or status query) */
 
 typedef struct vtx_server_config {
     uint64_t hbm_budget_bytes;     /* device bytes jobs may occupy; 0 = free HBM at creation */

@@ -220,5 +220,10 @@ def test_tuning_rejects_bad_values():
             s.set_tuning(gather_block=96)
         with pytest.raises(Exception):
             s.set_tuning(streams=5)
-        s.set_tuning(lra_split=0)
-        assert s.tuning()["lra_split"] == 0
+        for bad in (-1, 4):
+            with pytest.raises(Exception):
+                s.set_tuning(hash_grid=bad)
+        s.set_tuning(lra_split=0, hash_grid=3)
+        assert s.tuning()["lra_split"] == 0 and s.tuning()["hash_grid"] == 3
+        with pytest.raises(Exception):   # tune_hash_grid is 0 or 1
+            s.add_segment(S.Params(tune_hash_grid=2))
