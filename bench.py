@@ -140,7 +140,7 @@ def plain_rate(scenes, frames, stream):
     return dt
 
 
-def measured_traffic(workload, row_prefix="| `void k_iterate_grid<0,"):
+def measured_traffic(workload, row_prefix="| `void vtx::k_iterate_grid<0,"):
     """HBM bytes per launch of a kernel (the plain Jacobi pass by default) from the committed,
     copy-calibrated FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs),
     or None."""
