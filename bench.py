@@ -140,17 +140,24 @@ def plain_rate(scenes, frames, stream):
     return dt
 
 
-def measured_traffic(workload, row_prefix="| `void vtx::k_iterate_grid<0,"):
+def measured_traffic(workload, kernel="vtx::k_iterate_grid<0,"):
     """HBM bytes per launch of a kernel (the plain Jacobi pass by default) from the committed,
-    copy-calibrated FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs),
-    or None."""
+    copy-calibrated FETCH_SIZE / WRITE_SIZE summary of this workload (separate rocprofv3 runs):
+    the fetch and write columns of the kernel's row, or None when the file, the row or either
+    column is missing."""
     path = os.path.join(ROOT, "profiles", f"{workload}_hbm_counters.md")
     try:
+        cols = None
         for ln in open(path):
-            if ln.startswith(row_prefix):
-                cells = [c.strip() for c in ln.strip().strip("|").split("|")]
-                return (float(cells[1]) + float(cells[2])) * 1e6, os.path.relpath(path, ROOT)
-    except OSError:
+            if not ln.startswith("|"):
+                continue
+            cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+            if cells[0] == "kernel":
+                cols = ([k for k, c in enumerate(cells) if c.startswith("fetch MB")] +
+                        [k for k, c in enumerate(cells) if c.startswith("write MB")])
+            elif cols and len(cols) == 2 and cells[0].startswith(f"`void {kernel}"):
+                return (float(cells[cols[0]]) + float(cells[cols[1]])) * 1e6, os.path.relpath(path, ROOT)
+    except (OSError, ValueError):
         pass
     return None, None
 
