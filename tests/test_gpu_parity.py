@@ -232,9 +232,12 @@ def test_config1_sixty_frames_vs_fp64(stencil):
     say(f"  max err {err:.3e} (extent {sc.extent():.3f})")
     assert np.all(np.isfinite(x))
     assert err <= 1e-3 * sc.extent()
-    # normals agree too (unit vectors)
+    # normals agree too (unit vectors; measured on MI355X: 2.5e-4 after one substep and after 60
+    # frames, on both Jacobi paths)
     nref = ref.normals.reshape(-1, 3)
-    assert np.max(np.abs(nrm - nref)) < 5e-2
+    nerr = float(np.max(np.abs(nrm - nref)))
+    say(f"  normals max err {nerr:.3e}")
+    assert nerr < 2e-3
 
 
 @pytest.mark.parametrize("bend_mode", ["quad", "pairs", "none"])

@@ -832,7 +832,8 @@ static Ranges needs_of(const Segment& sg) {
     Ranges r;
     r.n_p = sg.n();
     r.n_cell = sg.cell_capacity();
-    r.n_nbr = sg.selfcol() ? (size_t)sg.prm.max_num_neighbors * sg.n() : 0;
+    // neighbour lists in 16-byte chunks of four entries (the cap rounded up to whole chunks)
+    r.n_nbr = sg.selfcol() ? (size_t)((sg.prm.max_num_neighbors + 3) & ~3) * sg.n() : 0;
     r.n_st = pr.est.size();
     r.n_at = pr.eat.size();
     r.n_bd = pr.ebd.size();
@@ -994,6 +995,9 @@ void Engine::upload(Segment& sg, hipStream_t s) {
     take(ap_, need.n_p, &r.p);
     take(acell_, need.n_cell, &r.cell);
     take(anbr_, need.n_nbr, &r.nbr);
+    // every neighbour range is a multiple of four entries, so every offset is one too: the lists'
+    // 16-byte chunk stores stay aligned
+    if (r.nbr % 4) throw Error(VTX_ESTATE, "neighbour table range not 16-byte aligned");
     take(ast_, need.n_st, &r.st);
     take(aat_, need.n_at, &r.at);
     take(abd_, need.n_bd, &r.bd);
@@ -1976,9 +1980,13 @@ void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint
     const uint32_t* vals = sorted_src_ ? pl_.v1 : pl_.v0;
     if (sh) VTX_HIP(hipMemcpy(sh, keys + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
     if (si) VTX_HIP(hipMemcpy(si, vals + d.pbase, sizeof(uint32_t) * d.n, hipMemcpyDeviceToHost));
-    if (nbr) {
+    if (nbr) {   // device chunks (entries 4c..4c+3 of particle i at 4 (c n + i)) to column-major
         const uint64_t nb = (uint64_t)d.nbase_hi << 32 | d.nbase_lo;
-        VTX_HIP(hipMemcpy(nbr, pl_.nbr + nb, sizeof(uint32_t) * (size_t)d.max_nbr * d.n, hipMemcpyDeviceToHost));
+        const size_t K = d.max_nbr, n = d.n, K4 = (K + 3) & ~size_t(3);
+        std::vector<uint32_t> tmp(K4 * n);
+        VTX_HIP(hipMemcpy(tmp.data(), pl_.nbr + nb, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < K; k++)
+            for (size_t i = 0; i < n; i++) nbr[k * n + i] = tmp[4 * ((k >> 2) * n + i) + (k & 3)];
     }
 }
 

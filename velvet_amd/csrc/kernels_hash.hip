@@ -433,10 +433,14 @@ __device__ __forceinline__ float4 ld_sorted(const float4* base, uint32_t i) {
 
 // The 27-cell walk of one particle (CacheNeighbors_Kernel, Velvet/SpatialHashGPU.cu:94-129): p / o
 // its predicted / initial position, id its index, sP the 32-B sorted records (position with
-// w = index at 2i, initial position at 2i + 1), cells the (start, end) table of M cells, nb its
-// column (entry k at nb[k * n]), shift the grid shift of the cell coordinates. crange is the
-// workgroup's LDS cache of cell ranges, one column per thread.
-template <int BS>
+// w = index at 2i, initial position at 2i + 1), cells the (start, end) table of M cells, shift
+// the grid shift of the cell coordinates. crange is the workgroup's LDS cache of cell ranges, one
+// column per thread. The list goes to nb, the particle's column of an n-particle table:
+//   CHUNK = false: entry k at nb[k * n] (column-major, the kernel-launch tier's layout);
+//   CHUNK = true:  entries in 16-byte chunks, chunk c = entries 4c..4c+3 at nb[4 (c * n)]
+//                  (nb = table + 4 id, 16-byte aligned): a lane's four consecutive entries share
+//                  one 16-byte run, and a reader loads four at once.
+template <int BS, bool CHUNK>
 __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, const uint32_t id,
                                               const float4* __restrict__ sP, const uint2* __restrict__ cells,
                                               uint32_t* __restrict__ nb, const uint32_t n, const float cell,
@@ -499,7 +503,7 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
         return s++;
     };
     // candidate test in walk order, one predicate and one predicated store at the running
-    // offset koff = k n; a full list takes no more entries
+    // offset koff of entry k; a full list takes no more entries
     uint32_t k = 0, koff = 0;
     auto take = [&](float4 qp, float4 r) {
         const uint32_t j = __float_as_uint(qp.w);
@@ -509,8 +513,8 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         const bool acc = (j != id) & (d2 < cell2) & (e2 > diam2) & (k < K);
         if (acc) nb[koff] = j;
+        koff += !acc ? 0u : !CHUNK ? n : (k & 3u) == 3u ? 4 * n - 3u : 1u;
         k += acc ? 1u : 0u;
-        koff += acc ? n : 0u;
     };
     // two candidates per trip with alternating load buffers: the loads of the next candidate
     // are in flight while the current one is tested
@@ -565,8 +569,8 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const float4 p = sorted ? sP[2 * i] : d.P[in][sg.pbase + i];
     const float4 o = sorted ? sX0[2 * i] : d.X0[sg.pbase + i];
     const uint32_t id = sorted ? __float_as_uint(p.w) : i;
-    uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + id;
-    neighbor_walk<BS>(p, o, id, sP, cells, nb, n, sg.cell, sg.cell2, sg.diam2, sg.M, seg_mod(sg), sg.max_nbr,
+    uint32_t* __restrict__ nb = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + 4 * (size_t)id;
+    neighbor_walk<BS, true>(p, o, id, sP, cells, nb, n, sg.cell, sg.cell2, sg.diam2, sg.M, seg_mod(sg), sg.max_nbr,
                       seg_shift(sg), crange);
 }
 
@@ -643,7 +647,7 @@ __global__ __launch_bounds__(BS) void k_raw_neighbors(const float4* __restrict__
     if (i >= n) return;
     const float4 p = rec[2 * (size_t)i], o = rec[2 * (size_t)i + 1];
     const uint32_t id = __float_as_uint(p.w);
-    neighbor_walk<BS>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, 0u, crange);
+    neighbor_walk<BS, false>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, 0u, crange);
 }
 
 // The interleaved table to the caller's cell_start / cell_end (empty cells: ~0u in both).

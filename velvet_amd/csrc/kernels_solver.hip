@@ -163,22 +163,30 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
     const V3 x = xyz(d.X[g]);
     if (sg.flags & VTX_SEG_SELFCOL) {
         const uint32_t n = sg.n;
-        const uint32_t* __restrict__ nb =
-            d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo) + (g - sg.pbase);
+        // the list in 16-byte chunks: entries 4c..4c+3 at chunk c, nb4[c * n]
+        const uint4* __restrict__ nb4 =
+            reinterpret_cast<const uint4*>(d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo)) + (g - sg.pbase);
         V3 delta = v3(0.f, 0.f, 0.f);
         int cnt = 0;
         const V3 vel_i = p - x;
-        // neighbours four at a time: while a group's positions load, the next group's indices
-        // load too; the four terms are then added in list order. An entry counts only if every
+        // neighbours four at a time (one chunk): while a group's positions load, the next chunk
+        // loads too; the four terms are then added in list order. An entry counts only if every
         // entry before it is a particle index (the list ends at the first sentinel, later rows are
         // stale). A neighbour's start-of-substep position enters only the friction of a contact,
         // so it is loaded for contacts alone: most listed neighbours of a resting cloth lie
         // beyond the diameter, and the pass is bound by the cache lines its gathers touch.
         bool live = true;
+        const uint32_t K = sg.max_nbr;
         uint32_t jn[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) jn[u] = (uint32_t)u < sg.max_nbr ? nb[(size_t)u * n] : VTX_INVALID;
-        for (uint32_t k0 = 0; k0 < sg.max_nbr && live; k0 += 4) {
+        auto chunk = [&](uint32_t k0) {   // entries k0..k0+3, those at K or beyond as sentinels
+            const uint4 c = nb4[(size_t)(k0 >> 2) * n];
+            jn[0] = c.x;
+            jn[1] = k0 + 1 < K ? c.y : VTX_INVALID;
+            jn[2] = k0 + 2 < K ? c.z : VTX_INVALID;
+            jn[3] = k0 + 3 < K ? c.w : VTX_INVALID;
+        };
+        chunk(0);
+        for (uint32_t k0 = 0; k0 < K && live; k0 += 4) {
             bool ok[4];
             uint32_t jg[4];
 #pragma unroll
@@ -190,9 +198,7 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
             float4 pj4[4], xj4[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) pj4[u] = P[jg[u]];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                jn[u] = live && k0 + 4 + u < sg.max_nbr ? nb[(size_t)(k0 + 4 + u) * n] : VTX_INVALID;
+            if (live && k0 + 4 < K) chunk(k0 + 4);
             bool hit[4];
             float dist[4];
 #pragma unroll

@@ -144,7 +144,7 @@ struct DevState {
     uint2* cells;             // per hash cell: (start, end) in sorted order, start ~0u = empty
     float4* sPX;              // per sorted position, 32 B: the predicted position with w = local
                               // particle index, then the initial position (one cache line holds both)
-    uint32_t* nbr;            // neighbour tables, column-major per segment
+    uint32_t* nbr;            // neighbour tables per segment, 16-byte chunks of four entries
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
     const GridTileDev* gtiles;
     const GridTileDev* ltiles;   // grid tiles whose dense attachments run on four lanes per particle
