@@ -472,7 +472,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
 // the tile is evaluated once (not once per member vertex), its four corrections are kept in
 // LDS, and each vertex gathers its 8 stretch terms, its attachments and its 12 bending terms in
 // a fixed order. The three slot types are evaluated one after another through the same LDS
-// area (X, then R, then D), which keeps the workgroup at ~21 KB of LDS.
+// area (X, then R, then D), which keeps the workgroup at ~18 KB of LDS (8 workgroups per CU).
 // Slot owner ranges relative to the tile origin (a0, b0): X rows/cols [-1, 15] (17 x 17),
 // R rows [-1, 16] cols [-1, 15] (18 x 17), D rows [-1, 15] cols [-1, 16] (17 x 18).
 #define VTX_NSX (17 * 17)
@@ -499,8 +499,10 @@ __device__ __forceinline__ void slot_rect(int a0, int b0, int rows, int cols, in
 }
 
 // Evaluate the bending slot owned at tile-relative cell (ra, rb) of one type (a cell of
-// slot_rect's rectangle) into LDS at its row-major index in the type's range; act (cleared
-// before) marks the slots that contribute.
+// slot_rect's rectangle) into LDS at its row-major index j in the type's range; act (cleared
+// before) marks the slots that contribute. The slot keeps g_k = -lam d_k for roles 0..2 (planes
+// 3k..3k+2); role 3's is -(g_0 + g_1 + g_2), as the four gradients of a dihedral angle sum to
+// zero, and each vertex scales the sum of its terms by its own inverse mass.
 template <int TYPE>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
                                           float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int cols,
@@ -527,12 +529,10 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
         BendGrad bg;
         on = bend_eval(q0, q1, q2, q3, rest, xpbd, lam, bg);
         if (on) {
-            const V3 c0 = (-q0.w * lam * bg.k0) * bg.c1, c1 = (-q1.w * lam * bg.k1) * bg.c2,
-                     c2 = (-q2.w * lam) * bg.d2, c3 = (-q3.w * lam) * bg.d3;
-            corr[0][j] = c0.x; corr[1][j] = c0.y; corr[2][j] = c0.z;
-            corr[3][j] = c1.x; corr[4][j] = c1.y; corr[5][j] = c1.z;
-            corr[6][j] = c2.x; corr[7][j] = c2.y; corr[8][j] = c2.z;
-            corr[9][j] = c3.x; corr[10][j] = c3.y; corr[11][j] = c3.z;
+            const V3 g0 = (-lam * bg.k0) * bg.c1, g1 = (-lam * bg.k1) * bg.c2, g2 = -lam * bg.d2;
+            corr[0][j] = g0.x; corr[1][j] = g0.y; corr[2][j] = g0.z;
+            corr[3][j] = g1.x; corr[4][j] = g1.y; corr[5][j] = g1.z;
+            corr[6][j] = g2.x; corr[7][j] = g2.y; corr[8][j] = g2.z;
             act[j] = 1;
         }
     }
@@ -541,12 +541,12 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
 // One slot type: the workgroup's threads take the cells of the type's rectangle densely (thread
 // t: row t / w, column t % w), so a tile cut by the cloth's edge runs no lanes on cells that
 // cannot hold a slot; then every tile vertex adds its four memberships of this type in role
-// order.
+// order to bend (gradients times -lam; the vertex's inverse mass is applied to the sum).
 template <int TYPE, int BT>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
                                                float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
-                                               V3& delta, int& cnt) {
+                                               V3& bend, int& cnt) {
     const uint32_t tid = threadIdx.x;
     if (!((sg.flags >> (6 + TYPE)) & 1u)) {   // the type is present at all
         int ra0, rb0, h, w;
@@ -583,7 +583,12 @@ __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& 
         for (int r = 0; r < 4; r++) {
             const int j = sidx[r];
             if (act[j]) {
-                delta += v3(corr[3 * r][j], corr[3 * r + 1][j], corr[3 * r + 2][j]);
+                if (r < 3) {
+                    bend += v3(corr[3 * r][j], corr[3 * r + 1][j], corr[3 * r + 2][j]);
+                } else {
+                    bend -= v3(corr[0][j], corr[1][j], corr[2][j]) + v3(corr[3][j], corr[4][j], corr[5][j]) +
+                            v3(corr[6][j], corr[7][j], corr[8][j]);
+                }
                 cnt++;
             }
         }
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
                                                                    int in, float h) {
     constexpr int BT = VTX_TILE * SPLIT;
     __shared__ float4 sp[VTX_GP * VTX_GP];
-    __shared__ float corr[12][VTX_NSMAX];          // plane r*3+c: component c of role r
+    __shared__ float corr[9][VTX_NSMAX];           // plane r*3+c: component c of role r's g_r
     __shared__ uint32_t actw[(VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4];   // per type: slot contributes
     uint8_t* const act = reinterpret_cast<uint8_t*>(actw);
     const GridTileDev t = tiles[tile_of(d.remap)];
@@ -681,11 +686,13 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     }
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
-    bend_type_pass<0, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
-    bend_type_pass<1, BT>(d, sg, sp, corr, act + VTX_NSX, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, delta, cnt);
+    V3 bend = v3(0.f, 0.f, 0.f);
+    bend_type_pass<0, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend, cnt);
+    bend_type_pass<1, BT>(d, sg, sp, corr, act + VTX_NSX, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend, cnt);
     bend_type_pass<2, BT>(d, sg, sp, corr, act + VTX_NSX + VTX_NSR, a0, b0, rows, cols, xpbd_bend, inside, ty, tx,
-                          delta, cnt);
+                          bend, cnt);
     if (!inside) return;
+    delta += w * bend;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
     epilogue<EPI>(d, sg, g, in, p, w, h);
 }
