@@ -261,9 +261,12 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 
 // The whole spatial-hash build of one small segment in one workgroup of VTX_HASH_LDS_BT threads:
 // keys of the predicted positions (k_hash_keys), a stable LSD radix sort in LDS on key << 16 |
-// local index (8 bits per pass; each wave counts and then scatters its own contiguous chunk, 64
-// keys at a time ranked by wave ballots, so only three workgroup barriers per pass), the cell
-// table (k_clear_cells + k_cell_bounds) and the sorted records.
+// local index (8 bits per pass; each wave counts its own contiguous chunk with LDS atomics and
+// then scatters it, 64 keys at a time ranked by wave ballots over the pass's digit bits, so only
+// three workgroup barriers per pass), the cell table (k_clear_cells + k_cell_bounds) and the
+// sorted records. The sort runs on one CU and is bound by its VALU issue (a wave64 instruction
+// takes four cycles of a 16-lane SIMD), hence the atomic counts: the ballot match is needed only
+// for the ranks of the stable scatter.
 // Same keys, same stable order, so the same table and records as the device-wide path, in one
 // launch for every small segment instead of a dozen over all of them.
 // NMAX: particles per segment at most (16384: 144 KB of LDS, one workgroup per CU). Global
@@ -309,21 +312,11 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
     uint32_t* __restrict__ hw = wcnt[wave];
     uint32_t cur = 0;
     for (uint32_t shift = 16; shift < 16 + bits; shift += 8) {
-        // 1. per-wave digit counts, 64 keys at a time (the same ballot match as below)
+        const int nb = (int)min(8u, 16u + bits - shift);   // digit bits of this pass (the key has `bits`)
+        // 1. per-wave digit counts (LDS atomics on the wave's own counters: order does not matter)
         for (uint32_t k = lane; k < 256; k += 64) hw[k] = 0;
-        for (uint32_t r0 = c0; r0 < c1; r0 += 64) {
-            const uint32_t i = r0 + lane;
-            const bool valid = i < c1;
-            const uint32_t dg = valid ? (buf[cur][i] >> shift) & 255u : 0u;
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const bool bit = (dg >> b) & 1u;
-                const uint64_t bal = __ballot(valid && bit);
-                peers &= bit ? bal : ~bal;
-            }
-            if (valid && (peers & lt) == 0) hw[dg] += (uint32_t)__popcll(peers);
-        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = c0 + lane; i < c1; i += 64) atomicAdd(&hw[(buf[cur][i] >> shift) & 255u], 1u);
         __syncthreads();
         // 2. each digit's start: digits in order, then waves in order (first 256 threads)
         uint32_t tot = 0, incl = 0;
@@ -349,12 +342,15 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
             const bool valid = i < c1;
             const uint32_t e = valid ? buf[cur][i] : 0u;
             const uint32_t dg = (e >> shift) & 255u;
-            uint64_t peers = __ballot(valid);
+            // lanes with this lane's digit: per digit bit, the ballot of the bit or its complement
+            // (invalid lanes hold digit 0 and are cleared by the first mask)
+            uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
             for (int b = 0; b < 8; b++) {
-                const bool bit = (dg >> b) & 1u;
-                const uint64_t bal = __ballot(valid && bit);
-                peers &= bit ? bal : ~bal;
+                if (b >= nb) break;                 // bits above the key are zero in every digit
+                const uint32_t m = (uint32_t)((int32_t)(dg << (31 - b)) >> 31);   // bit b as 0 / ~0
+                const uint64_t bal = __builtin_amdgcn_ballot_w64(m != 0u);
+                peers &= ~(bal ^ ((uint64_t)m << 32 | m));   // lanes whose bit b equals this lane's
             }
             const uint32_t rank = (uint32_t)__popcll(peers & lt);
             const uint32_t at = hw[dg];
