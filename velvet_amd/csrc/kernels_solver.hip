@@ -479,6 +479,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
 #define VTX_NSR (18 * 17)
 #define VTX_NSD (17 * 18)
 #define VTX_NSMAX VTX_NSR
+#define VTX_RC 18              // owner cells of any type per tile side: -1..16
 
 template <int TYPE>
 __device__ __forceinline__ int slot_index(int ra, int rb) {
@@ -502,11 +503,13 @@ __device__ __forceinline__ void slot_rect(int a0, int b0, int rows, int cols, in
 // slot_rect's rectangle) into LDS at its row-major index j in the type's range; act (cleared
 // before) marks the slots that contribute. The slot keeps g_k = -lam d_k for roles 0..2 (planes
 // 3k..3k+2); role 3's is -(g_0 + g_1 + g_2), as the four gradients of a dihedral angle sum to
-// zero, and each vertex scales the sum of its terms by its own inverse mass.
-template <int TYPE>
+// zero, and each vertex scales the sum of its terms by its own inverse mass. SREST: the rest
+// angles come from srest, the tile's owner cells (rows and columns -1..16, row-major by 18) staged
+// in LDS with the positions, instead of from brest4.
+template <int TYPE, bool SREST>
 __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, const float4* sp,
-                                          float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0, int cols,
-                                          int ra, int rb, float xpbd) {
+                                          const float4* srest, float (*corr)[VTX_NSMAX], uint8_t* act, int a0,
+                                          int b0, int cols, int ra, int rb, float xpbd) {
     constexpr int W = TYPE == 2 ? 18 : 17;                                  // owner columns of the range
     const uint32_t j = (uint32_t)((ra + 1) * W + (rb + 1));                 // row-major storage index
     const int a = a0 + ra, b = b0 + rb;
@@ -522,7 +525,7 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
         const float4 q0 = sp[o + i0], q1 = sp[o + i1], q2 = sp[o + i2], q3 = sp[o + i3];
         float rest = 0.f;
         if (!(sg.flags & VTX_SEG_BEND_REST0)) {
-            const float4 r4 = d.brest4[sg.pbase + (uint32_t)(a * cols + b)];
+            const float4 r4 = SREST ? srest[(ra + 1) * VTX_RC + (rb + 1)] : d.brest4[sg.pbase + (uint32_t)(a * cols + b)];
             rest = TYPE == 0 ? r4.x : TYPE == 1 ? r4.y : r4.z;
         }
         float lam;
@@ -542,9 +545,10 @@ __device__ __forceinline__ void eval_slot(const DevState& d, const SegDev& sg, c
 // t: row t / w, column t % w), so a tile cut by the cloth's edge runs no lanes on cells that
 // cannot hold a slot; then every tile vertex adds its four memberships of this type in role
 // order to bend (gradients times -lam; the vertex's inverse mass is applied to the sum).
-template <int TYPE, int BT>
+template <int TYPE, int BT, bool SREST>
 __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& sg, const float4* sp,
-                                               float (*corr)[VTX_NSMAX], uint8_t* act, int a0, int b0,
+                                               const float4* srest, float (*corr)[VTX_NSMAX], uint8_t* act,
+                                               int a0, int b0,
                                                int rows, int cols, float xpbd, bool inside, int ty, int tx,
                                                V3& bend, int& cnt) {
     const uint32_t tid = threadIdx.x;
@@ -559,7 +563,7 @@ __device__ __forceinline__ void bend_type_pass(const DevState& d, const SegDev& 
                 const uint32_t t = tid + r * BT;
                 if (t < n) {
                     const uint32_t q = (t * m) >> 16;
-                    eval_slot<TYPE>(d, sg, sp, corr, act, a0, b0, cols, ra0 + (int)q, rb0 + (int)(t - q * (uint32_t)w), xpbd);
+                    eval_slot<TYPE, SREST>(d, sg, sp, srest, corr, act, a0, b0, cols, ra0 + (int)q, rb0 + (int)(t - q * (uint32_t)w), xpbd);
                 }
             }
         }
@@ -606,6 +610,9 @@ __device__ __forceinline__ float4 nan4() {
 // tile per CU, and a single wave per SIMD would walk 256 slots per vertex): thread q of a vertex
 // sums attachment part q, the first thread adds the parts in order, and all threads share the
 // bending slots (one round per type). Both forms add the same terms in the same order.
+// The split form (one workgroup per CU on a large cloth, so little hides a load's latency) also
+// stages the tile's bending rest angles in LDS with the positions instead of loading one per slot
+// evaluation.
 template <int EPI, int LB, int SPLIT>
 __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, const GridTileDev* __restrict__ tiles,
                                                                    int in, float h) {
@@ -613,6 +620,8 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     __shared__ float4 sp[VTX_GP * VTX_GP];
     __shared__ float corr[9][VTX_NSMAX];           // plane r*3+c: component c of role r's g_r
     __shared__ uint32_t actw[(VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4];   // per type: slot contributes
+    constexpr bool PRE = SPLIT > 1;
+    __shared__ float4 srest_s[PRE ? VTX_RC * VTX_RC : 1];
     uint8_t* const act = reinterpret_cast<uint8_t*>(actw);
     const GridTileDev t = tiles[tile_of(d.remap)];
     const SegDev sg = d.segs[t.seg];
@@ -620,24 +629,46 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     const int a0 = (int)t.a0, b0 = (int)t.b0;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = SPLIT == 1 ? tid : tid % VTX_TILE;
-    const uint32_t qpart = SPLIT == 1 ? 0u : tid / VTX_TILE;
+    // part of the split form: wave-uniform (VTX_TILE is a multiple of the wave), so the slot
+    // positions below are scalar loads
+    const uint32_t qpart = SPLIT == 1 ? 0u : __builtin_amdgcn_readfirstlane(tid / VTX_TILE);
     const float4* __restrict__ P = d.P[in];
-
-    // clear the slot marks; stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
-    for (uint32_t e = tid; e < (VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4; e += BT) actw[e] = 0u;
-    for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += BT) {
-        const int a = a0 - VTX_GH + (int)(e / VTX_GP), b = b0 - VTX_GH + (int)(e % VTX_GP);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (a >= 0 && a < rows && b >= 0 && b < cols) v = P[sg.pbase + (uint32_t)(a * cols + b)];
-        sp[e] = v;
-    }
-    __syncthreads();
-
     const int ty = (int)(lane / VTX_GT), tx = (int)(lane % VTX_GT);
     const int a = a0 + ty, b = b0 + tx;
     const bool in_cloth = a < rows && b < cols;
     const bool inside = in_cloth && qpart == 0;        // the thread that owns the vertex
     const uint32_t g = in_cloth ? sg.pbase + (uint32_t)(a * cols + b) : 0u;
+    // stretch rest lengths: R own/left, D own/up, X-diagonal own/up-left, anti-diagonal of left/up,
+    // issued before the staging loads and always (from the vertex itself where a neighbour is
+    // missing): a load that a lane may skip leaves the outstanding count unknown to the compiler,
+    // which then waits for every load before the first use
+    const float4 r_own = d.rest4[g];
+    float4 r_left = d.rest4[inside && b > 0 ? g - 1 : g];
+    float4 r_up = d.rest4[inside && a > 0 ? g - cols : g];
+    float4 r_ul = d.rest4[inside && a > 0 && b > 0 ? g - cols - 1 : g];
+    if (!(b > 0)) r_left = nan4();
+    if (!(a > 0)) r_up = nan4();
+    if (!(a > 0 && b > 0)) r_ul = nan4();
+
+    // clear the slot marks; stage positions of rows [a0-2, a0+18) x cols [b0-2, b0+18)
+    for (uint32_t e = tid; e < (VTX_NSX + VTX_NSR + VTX_NSD + 3) / 4; e += BT) actw[e] = 0u;
+    for (uint32_t e = tid; e < VTX_GP * VTX_GP; e += BT) {
+        const int sa = a0 - VTX_GH + (int)(e / VTX_GP), sb = b0 - VTX_GH + (int)(e % VTX_GP);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (sa >= 0 && sa < rows && sb >= 0 && sb < cols) v = P[sg.pbase + (uint32_t)(sa * cols + sb)];
+        sp[e] = v;
+    }
+    if (PRE && !(sg.flags & VTX_SEG_BEND_REST0)) {   // rest angles of owner cells rows/cols -1..16
+        for (uint32_t e = tid; e < VTX_RC * VTX_RC; e += BT) {
+            const int sa = a0 - 1 + (int)(e / VTX_RC), sb = b0 - 1 + (int)(e % VTX_RC);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sa >= 0 && sa < rows && sb >= 0 && sb < cols) v = d.brest4[sg.pbase + (uint32_t)(sa * cols + sb)];
+            srest_s[e] = v;
+        }
+    }
+
+    __syncthreads();
+
     const int sc = (ty + VTX_GH) * VTX_GP + (tx + VTX_GH);    // staged index of this vertex
     const float4 p4 = sp[sc];
     V3 p = xyz(p4);
@@ -646,11 +677,6 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     int cnt = 0;
 
     if (inside) {
-        // stretch: R own/left, D own/up, X-diagonal own/up-left, anti-diagonal of left/up
-        const float4 r_own = d.rest4[g];
-        const float4 r_left = b > 0 ? d.rest4[g - 1] : nan4();
-        const float4 r_up = a > 0 ? d.rest4[g - cols] : nan4();
-        const float4 r_ul = (a > 0 && b > 0) ? d.rest4[g - cols - 1] : nan4();
         const float k = sg.stiffness;
         if (r_own.x == r_own.x) stretch_term(p, w, sp[sc + 1], r_own.x, k, delta, cnt);
         if (r_left.x == r_left.x) stretch_term(p, w, sp[sc - 1], r_left.x, k, delta, cnt);
@@ -669,7 +695,7 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
         if (in_cloth) {
             uint32_t s0, s1;
             lra_part_range(sg, qpart, s0, s1);
-            attach_dense<LB>(d, sg, (uint32_t)(a * cols + b), s0, s1, p, w, part, pc);
+            attach_dense<LB>(d, sg, g - sg.pbase, s0, s1, p, w, part, pc);
             if (qpart > 0) lpart[qpart - 1][lane] = f4(part, __int_as_float(pc));
         }
         __syncthreads();
@@ -687,10 +713,12 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     // bending: X roles 0..3, R roles 0..3, D roles 0..3
     const float xpbd_bend = sg.bend_compliance * frcp(h) * frcp(h);   // compliance / h^2 (cu:173)
     V3 bend = v3(0.f, 0.f, 0.f);
-    bend_type_pass<0, BT>(d, sg, sp, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend, cnt);
-    bend_type_pass<1, BT>(d, sg, sp, corr, act + VTX_NSX, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend, cnt);
-    bend_type_pass<2, BT>(d, sg, sp, corr, act + VTX_NSX + VTX_NSR, a0, b0, rows, cols, xpbd_bend, inside, ty, tx,
-                          bend, cnt);
+    const float4* const srest = srest_s;   // read by the split form (PRE) only, when staged
+    bend_type_pass<0, BT, PRE>(d, sg, sp, srest, corr, act, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend, cnt);
+    bend_type_pass<1, BT, PRE>(d, sg, sp, srest, corr, act + VTX_NSX, a0, b0, rows, cols, xpbd_bend, inside, ty, tx, bend,
+                          cnt);
+    bend_type_pass<2, BT, PRE>(d, sg, sp, srest, corr, act + VTX_NSX + VTX_NSR, a0, b0, rows, cols, xpbd_bend, inside, ty,
+                          tx, bend, cnt);
     if (!inside) return;
     delta += w * bend;
     if (cnt > 0) p += delta * (frcp((float)cnt) * sg.omega);
