@@ -110,10 +110,13 @@ def _run_pair(sc, v0, frames, frame_dt=None, stencil_expected=None):
     return xg, sim.x.reshape(-1, 3)
 
 
-@pytest.mark.parametrize("kind", ["stacked", "folded"])
-def test_contacts_one_substep_vs_fp64(kind):
+@pytest.mark.parametrize("kind,K", [("stacked", 64), ("folded", 64), ("stacked", 13), ("folded", 7)])
+def test_contacts_one_substep_vs_fp64(kind, K):
+    """K = max_num_neighbors: 13 and 7 end the device lists in a partial chunk of four entries
+    (the contact pass masks the entries past the cap)."""
     sc, moving = stacked() if kind == "stacked" else folded()
     sc.params.num_substeps = 1
+    sc.params.max_num_neighbors = K
     h = F32(S.FRAME_DT / F32(5))
     d = float(sc.params.particle_diameter)
     vdown = -0.07 * d / float(h)
@@ -128,7 +131,7 @@ def test_contacts_one_substep_vs_fp64(kind):
     x0 = S.flatten(sc)["x"].astype(np.float64)
     v0 = _start_velocities(sc, moving, vdown, amp=1e-2)
     pairs, parts = _contact_pairs(sc, x0, v0.astype(np.float64), float(h))
-    say(f"contacts {kind}: {pairs} contact pairs, {parts} of {sc.n} particles in contact at substep 0")
+    say(f"contacts {kind} (K={K}): {pairs} contact pairs, {parts} of {sc.n} particles in contact at substep 0")
     assert parts >= 0.1 * sc.n and pairs >= 500
     xg, xr = _run_pair(sc, v0, 1, frame_dt=h, stencil_expected=1 if kind == "folded" else 0)
     err = float(np.max(np.abs(xg - xr)))

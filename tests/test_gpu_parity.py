@@ -172,25 +172,34 @@ def test_hash_stage_matches_golden_fixture():
     s.close()
 
 
-def test_hash_caps_and_duplicates():
-    say("hash caps/duplicates")
+@pytest.mark.parametrize("K", [64, 13, 7, 1])
+def test_hash_caps_and_duplicates(K):
+    """80 particles: the first 50 share one predicted position (49 candidates each, lists cut at
+    the cap), the other 30 are isolated (empty lists) or in a group of 10 (9 entries and a
+    sentinel when the cap is above 9); caps that are not a multiple of four end in a partial
+    16-byte chunk of the device table."""
+    say(f"hash caps/duplicates, max_num_neighbors {K}")
     n = 80
     x0 = np.stack([np.arange(n) * 1.0, np.zeros(n), np.zeros(n)], 1).astype(F32)
     p = np.tile(np.array([[0.01, 0.01, 0.01]], F32), (n, 1))
+    p[50:60] = (5.01, 0.01, 0.01)
+    p[60:] = np.stack([10.0 + np.arange(20) * 3.0, np.zeros(20), np.zeros(20)], 1)
     c = S.Cloth(x=x0, tris=np.zeros((0, 3), np.uint32), stretch=np.zeros((0, 2), np.int32),
                 stretch_rest=np.zeros(0, F32), bend=np.zeros((0, 4), np.uint32), bend_rest=np.zeros(0, F32),
                 slots=np.zeros((0, 3), F32), attach_pid=np.zeros(0, np.int32), attach_slot=np.zeros(0, np.int32),
                 attach_dist=np.zeros(0, F32), diameter=F32(0.05))
     for M in (2 * n, 1, 5):
         s = ClothSolver(use_graph=False)
-        prm = S.Params(hash_table_size=M, particle_diameter=0.05, hash_cell_size_scalar=2.0)
+        prm = S.Params(hash_table_size=M, particle_diameter=0.05, hash_cell_size_scalar=2.0, max_num_neighbors=K)
         s.set_params(prm)
         s.AddCloth(c.x, c.tris, c.diameter)
         s.set_params(prm)
         s.write_predicted(p)
         s.run_hash()
-        ref = oracle.hash32(p, x0, 0.1, 0.05, M)
-        _check_hash_equal(s.read_hash(0), ref, n)
+        ref = oracle.hash32(p, x0, 0.1, 0.05, M, max_nbr=K)
+        got = s.read_hash(0)
+        assert got[4].shape == (K, n)
+        _check_hash_equal(got, ref, n)
         s.close()
 
 
