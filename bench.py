@@ -424,7 +424,7 @@ def main():
     ap.add_argument("--serve-pipelined", type=int, default=1,
                     help="1: each serving step returns once its frame is launched, so request handling "
                          "overlaps the frame; 0: each step waits for its frame")
-    ap.add_argument("--serve-tuning", default="neighbor_order=2,hash_lds=0,streams=2,hash_grid=1", metavar="K=V,...",
+    ap.add_argument("--serve-tuning", default="neighbor_order=2,hash_lds=0,streams=2,hash_grid=1,lra_batch=4", metavar="K=V,...",
                     help="launch variants of the serving solver (the live set changes every step, so it is "
                          "not tuned online); the default is the measured best of a sweep on this job mix")
     ap.add_argument("--no-serve", action="store_true")
