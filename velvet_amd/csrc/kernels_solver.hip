@@ -232,15 +232,14 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
 enum { EPI_NONE = 0, EPI_FINALIZE = 1, EPI_FINALIZE_PREDICT = 2 };
 
 // One stretch constraint seen from one endpoint (Velvet/VtClothSolverGPU.cu:80-98): bilateral,
-// compliance 0, eps in the gradient denominator. q = other endpoint (xyz, inverse mass).
-__device__ __forceinline__ void stretch_term(V3 p, float w, float4 q, float rest, float stiff, V3& delta,
-                                             int& cnt) {
+// compliance 0, eps in the gradient denominator. q = other endpoint (xyz, inverse mass);
+// nws = -w * stiffness, the endpoint's factor of every one of its terms.
+__device__ __forceinline__ void stretch_term(V3 p, float w, float nws, float4 q, float rest, V3& delta, int& cnt) {
     const V3 diff = p - xyz(q);
     const float dist = fsqrt(dot(diff, diff));
     const float wsum = w + q.w;
     if (dist != rest && wsum > 0.f) {
-        const float s = (dist - rest) * stiff * frcp(wsum * (dist + VTX_EPSILON));
-        delta += (-w * s) * diff;
+        delta += ((dist - rest) * nws * frcp(wsum * (dist + VTX_EPSILON))) * diff;
         cnt++;
     }
 }
@@ -359,7 +358,7 @@ __device__ __forceinline__ void attach_terms(const DevState& d, const SegDev& sg
 // Returns false on the reference's early-outs; else lam carries the sign rule and the correction
 // of vertex k is -w_k * lam * d_k.
 struct BendGrad {
-    V3 c1, c2, d2, d3;
+    V3 c1, c2, d2;   // d0 = k0 c1, d1 = k1 c2, d3 = -(d0 + d1 + d2)
     float k0, k1;
 };
 
@@ -374,17 +373,20 @@ __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float
     const V3 a = p2 - p0, b = p3 - p1;
     g.c1 = cross(a, e);
     g.c2 = cross(e, b);
-    const float r1 = frsq(dot(g.c1, g.c1)), r2 = frsq(dot(g.c2, g.c2));
+    const float g11 = dot(g.c1, g.c1), g22 = dot(g.c2, g.c2), g12 = dot(g.c1, g.c2);
+    const float r1 = frsq(g11), r2 = frsq(g22);
     const float r1s = r1 * r1, r2s = r2 * r2;
     g.k0 = elen * r1s;
     g.k1 = elen * r2s;
     const float s1 = ((-dot(a, e) - ee) * inv_elen) * r1s, s2 = (-dot(b, e) * inv_elen) * r2s;
     g.d2 = s1 * g.c1 + s2 * g.c2;
-    g.d3 = (-(g.k0 + s1)) * g.c1 + (-(g.k1 + s2)) * g.c2;
-    const float cd = fminf(fmaxf(dot(g.c1, g.c2) * (r1 * r2), -1.0f), 1.0f);
+    const float cd = fminf(fmaxf(g12 * (r1 * r2), -1.0f), 1.0f);
     const float phi = facos(cd);
-    // |d0|^2 = |e|^2 r1^2 and |d1|^2 = |e|^2 r2^2
-    float l = ee * (q0.w * r1s + q1.w * r2s) + q2.w * dot(g.d2, g.d2) + q3.w * dot(g.d3, g.d3);
+    // |d0|^2 = |e|^2 r1^2, |d1|^2 = |e|^2 r2^2, and d3 = t1 c1 + t2 c2 from the Gram entries:
+    // |d3|^2 = t1 (t1 g11 + 2 t2 g12) + t2^2 g22
+    const float t1 = -(g.k0 + s1), t2 = -(g.k1 + s2), t2g = t2 * g12;
+    const float d3s = __builtin_fmaf(t1, __builtin_fmaf(t1, g11, t2g + t2g), t2 * (t2 * g22));
+    float l = ee * (q0.w * r1s + q1.w * r2s) + q2.w * dot(g.d2, g.d2) + q3.w * d3s;
     if (l < VTX_EPSILON) return false;
     l = (phi - rest) * frcp(l + xpbd);
     if (dot(a, g.c2) > 0.0f) l = -l;      // (n1 x n2).e > 0
@@ -437,10 +439,11 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
         V3 delta = v3(0.f, 0.f, 0.f);
         int cnt = 0;
         const uint2* __restrict__ st = d.st + t.st_off + lane;
+        const float nws = -w * sg.stiffness;
         for (uint32_t k = 0; k < t.st_k; k++) {
             const uint2 e = st[(size_t)k * VTX_TILE];
             if (e.x == VTX_INVALID) continue;
-            stretch_term(p, w, P[sg.pbase + e.x], __uint_as_float(e.y), sg.stiffness, delta, cnt);
+            stretch_term(p, w, nws, P[sg.pbase + e.x], __uint_as_float(e.y), delta, cnt);
         }
         attach_terms<LB>(d, sg, t.at_off, t.at_k, lane, g - sg.pbase, p, w, delta, cnt);
         const uint32_t* __restrict__ bd = d.bd + t.bd_off + lane;
@@ -456,7 +459,8 @@ __global__ __launch_bounds__(VTX_TILE) void k_iterate(DevState d, const TileDev*
             BendGrad bg;
             if (!bend_eval(P[pb + ix.x], P[pb + ix.y], P[pb + ix.z], P[pb + ix.w], d.brest[bid], xpbd_bend, lam, bg))
                 continue;
-            const V3 dk = role == 0 ? bg.k0 * bg.c1 : role == 1 ? bg.k1 * bg.c2 : role == 2 ? bg.d2 : bg.d3;
+            const V3 dk = role == 0 ? bg.k0 * bg.c1 : role == 1 ? bg.k1 * bg.c2 : role == 2 ? bg.d2
+                                                                              : -(bg.k0 * bg.c1 + bg.k1 * bg.c2 + bg.d2);
             delta += (-w * lam) * dk;
             cnt++;
         }
@@ -677,15 +681,15 @@ __global__ __launch_bounds__(VTX_TILE * SPLIT) void k_iterate_grid(DevState d, c
     int cnt = 0;
 
     if (inside) {
-        const float k = sg.stiffness;
-        if (r_own.x == r_own.x) stretch_term(p, w, sp[sc + 1], r_own.x, k, delta, cnt);
-        if (r_left.x == r_left.x) stretch_term(p, w, sp[sc - 1], r_left.x, k, delta, cnt);
-        if (r_own.y == r_own.y) stretch_term(p, w, sp[sc + VTX_GP], r_own.y, k, delta, cnt);
-        if (r_up.y == r_up.y) stretch_term(p, w, sp[sc - VTX_GP], r_up.y, k, delta, cnt);
-        if (r_own.z == r_own.z) stretch_term(p, w, sp[sc + VTX_GP + 1], r_own.z, k, delta, cnt);
-        if (r_ul.z == r_ul.z) stretch_term(p, w, sp[sc - VTX_GP - 1], r_ul.z, k, delta, cnt);
-        if (r_left.w == r_left.w) stretch_term(p, w, sp[sc + VTX_GP - 1], r_left.w, k, delta, cnt);
-        if (r_up.w == r_up.w) stretch_term(p, w, sp[sc - VTX_GP + 1], r_up.w, k, delta, cnt);
+        const float k = -w * sg.stiffness;
+        if (r_own.x == r_own.x) stretch_term(p, w, k, sp[sc + 1], r_own.x, delta, cnt);
+        if (r_left.x == r_left.x) stretch_term(p, w, k, sp[sc - 1], r_left.x, delta, cnt);
+        if (r_own.y == r_own.y) stretch_term(p, w, k, sp[sc + VTX_GP], r_own.y, delta, cnt);
+        if (r_up.y == r_up.y) stretch_term(p, w, k, sp[sc - VTX_GP], r_up.y, delta, cnt);
+        if (r_own.z == r_own.z) stretch_term(p, w, k, sp[sc + VTX_GP + 1], r_own.z, delta, cnt);
+        if (r_ul.z == r_ul.z) stretch_term(p, w, k, sp[sc - VTX_GP - 1], r_ul.z, delta, cnt);
+        if (r_left.w == r_left.w) stretch_term(p, w, k, sp[sc + VTX_GP - 1], r_left.w, delta, cnt);
+        if (r_up.w == r_up.w) stretch_term(p, w, k, sp[sc - VTX_GP + 1], r_up.w, delta, cnt);
         if constexpr (SPLIT == 1) attach_terms<LB>(d, sg, t.at_off, t.at_k, tid, (uint32_t)(a * cols + b), p, w, delta, cnt);
     }
     if constexpr (SPLIT > 1) {   // only dense segments come here (VTX_SEG_LRA_DENSE)
