@@ -995,8 +995,8 @@ void Engine::upload(Segment& sg, hipStream_t s) {
     take(ap_, need.n_p, &r.p);
     take(acell_, need.n_cell, &r.cell);
     take(anbr_, need.n_nbr, &r.nbr);
-    // every neighbour range is a multiple of four entries, so every offset is one too: the lists'
-    // 16-byte chunk stores stay aligned
+    // every neighbour range is a multiple of four entries, so every offset is one too: the contact
+    // pass's 16-byte chunk loads stay aligned
     if (r.nbr % 4) throw Error(VTX_ESTATE, "neighbour table range not 16-byte aligned");
     take(ast_, need.n_st, &r.st);
     take(aat_, need.n_at, &r.at);

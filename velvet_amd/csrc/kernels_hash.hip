@@ -407,8 +407,8 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
 //   NBR_SORTED:   hash-sorted order, so the lanes of one cell walk identical candidate ranges and
 //                 their loads merge;
 //   NBR_PATCH:    16 x 16 vertex patches of a grid cloth (its stencil tiles): a wave is 4 rows x
-//                 16 columns, so lanes share cells and candidates, and each row of the
-//                 column-major table is written as four 64-byte runs.
+//                 16 columns, so lanes share cells and candidates, and the 16 lanes of a patch
+//                 row hold consecutive particles, whose chunks are consecutive in the table.
 // A first sweep over the 27 cells finds the non-empty ones (bit q of mask) and the total
 // candidate count, and keeps the ranges of the first VTX_NB_CACHE non-empty cells in this
 // lane's LDS column, packed as start | length << 25 (self-collision segments have < 2^25
