@@ -832,8 +832,10 @@ static Ranges needs_of(const Segment& sg) {
     Ranges r;
     r.n_p = sg.n();
     r.n_cell = sg.cell_capacity();
-    // neighbour lists in 16-byte chunks of four entries (the cap rounded up to whole chunks)
-    r.n_nbr = sg.selfcol() ? (size_t)((sg.prm.max_num_neighbors + 3) & ~3) * sg.n() : 0;
+    // neighbour lists in chunks of four entries (the cap rounded up to whole chunks), 32-bit
+    // entries, or 16-bit ones (VTX_SEG_NBR16) in a range rounded up to whole 16-byte units
+    const size_t k4n = (size_t)((sg.prm.max_num_neighbors + 3) & ~3) * sg.n();
+    r.n_nbr = !sg.selfcol() ? 0 : sg.n() <= VTX_NBR16_N ? ((k4n / 2 + 3) & ~size_t(3)) : k4n;
     r.n_st = pr.est.size();
     r.n_at = pr.eat.size();
     r.n_bd = pr.ebd.size();
@@ -995,8 +997,8 @@ void Engine::upload(Segment& sg, hipStream_t s) {
     take(ap_, need.n_p, &r.p);
     take(acell_, need.n_cell, &r.cell);
     take(anbr_, need.n_nbr, &r.nbr);
-    // every neighbour range is a multiple of four entries, so every offset is one too: the contact
-    // pass's 16-byte chunk loads stay aligned
+    // every neighbour range is a multiple of four words, so every offset is one too: the contact
+    // pass's chunk loads stay aligned
     if (r.nbr % 4) throw Error(VTX_ESTATE, "neighbour table range not 16-byte aligned");
     take(ast_, need.n_st, &r.st);
     take(aat_, need.n_at, &r.at);
@@ -1031,7 +1033,7 @@ void Engine::upload(Segment& sg, hipStream_t s) {
         d.bbase = (uint32_t)r.brec;
         d.trbase = (uint32_t)r.trec;
         d.flags = (selfcol ? VTX_SEG_SELFCOL : 0u) | (pr.grid ? pr.grid_flags : 0u) |
-                  (pr.lra_dense ? VTX_SEG_LRA_DENSE : 0u);
+                  (pr.lra_dense ? VTX_SEG_LRA_DENSE : 0u) | (selfcol && n <= VTX_NBR16_N ? VTX_SEG_NBR16 : 0u);
         d.lbase = (uint32_t)r.lra;
         d.max_nbr = (uint32_t)p.max_num_neighbors;
         // binary32 products exactly as the reference forms them (Velvet/SpatialHashGPU.hpp:21,44,47)
@@ -1983,10 +1985,20 @@ void Engine::read_hash(int32_t s, uint32_t* cs, uint32_t* ce, uint32_t* sh, uint
     if (nbr) {   // device chunks (entries 4c..4c+3 of particle i at 4 (c n + i)) to column-major
         const uint64_t nb = (uint64_t)d.nbase_hi << 32 | d.nbase_lo;
         const size_t K = d.max_nbr, n = d.n, K4 = (K + 3) & ~size_t(3);
-        std::vector<uint32_t> tmp(K4 * n);
-        VTX_HIP(hipMemcpy(tmp.data(), pl_.nbr + nb, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < K; k++)
-            for (size_t i = 0; i < n; i++) nbr[k * n + i] = tmp[4 * ((k >> 2) * n + i) + (k & 3)];
+        if (d.flags & VTX_SEG_NBR16) {
+            std::vector<uint16_t> tmp(K4 * n);
+            VTX_HIP(hipMemcpy(tmp.data(), pl_.nbr + nb, sizeof(uint16_t) * tmp.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < K; k++)
+                for (size_t i = 0; i < n; i++) {
+                    const uint16_t v = tmp[4 * ((k >> 2) * n + i) + (k & 3)];
+                    nbr[k * n + i] = v == 0xFFFFu ? VTX_INVALID : v;
+                }
+        } else {
+            std::vector<uint32_t> tmp(K4 * n);
+            VTX_HIP(hipMemcpy(tmp.data(), pl_.nbr + nb, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < K; k++)
+                for (size_t i = 0; i < n; i++) nbr[k * n + i] = tmp[4 * ((k >> 2) * n + i) + (k & 3)];
+        }
     }
 }
 

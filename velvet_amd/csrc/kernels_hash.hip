@@ -473,13 +473,25 @@ __device__ __forceinline__ NbCells neighbor_cells(const float4 p, const uint2* _
 // 2i + 1), cells the (start, end) table of M cells, shift the grid shift of the cell coordinates.
 // The list goes to nb, the particle's column of an n-particle table:
 //   CHUNK = false: entry k at nb[k * n] (column-major, the kernel-launch tier's layout);
-//   CHUNK = true:  entries in 16-byte chunks, chunk c = entries 4c..4c+3 at nb[4 (c * n)]
-//                  (nb = table + 4 id, 16-byte aligned): a lane's four consecutive entries share
-//                  one 16-byte run, and a reader loads four at once.
+//   CHUNK = true:  entries in chunks of four, chunk c = entries 4c..4c+3 at nb[4 (c * n)]
+//                  (nb = table + 4 id, chunk-aligned): a lane's four consecutive entries share
+//                  one 16-byte (T = uint32_t) or 8-byte (T = uint16_t) run, and a reader loads
+//                  four at once.
+// Entries are 32-bit or, for a segment with VTX_SEG_NBR16, 16-bit (the sentinel truncated to
+// 0xffff); nb.put(k, v) stores entry offset k.
+struct NbOut {
+    char* base;   // the particle's first entry
+    bool w16;     // 16-bit entries (uniform over the workgroup's segment)
+    __device__ __forceinline__ void put(uint32_t k, uint32_t v) const {
+        if (w16) *reinterpret_cast<uint16_t*>(base + ((size_t)k << 1)) = (uint16_t)v;
+        else *reinterpret_cast<uint32_t*>(base + ((size_t)k << 2)) = v;
+    }
+};
+
 template <int BS, bool CHUNK>
 __device__ __forceinline__ void neighbor_list(NbCells c, const float4 p, const float4 o, const uint32_t id,
                                               const float4* __restrict__ sP, const uint2* __restrict__ cells,
-                                              uint32_t* __restrict__ nb, const uint32_t n, const float cell,
+                                              const NbOut nb, const uint32_t n, const float cell,
                                               const float cell2, const float diam2, const uint32_t M,
                                               const uint64_t mc, const uint32_t K, const uint32_t shift,
                                               const uint32_t (&crange)[VTX_NB_CACHE][BS], const uint32_t col) {
@@ -488,7 +500,7 @@ __device__ __forceinline__ void neighbor_list(NbCells c, const float4 p, const f
     const uint32_t total = c.total;
     uint32_t mask = c.mask;
     if (total == 0) {
-        nb[0] = VTX_INVALID;
+        nb.put(0, VTX_INVALID);
         return;
     }
     // position of the next candidate (called only while candidates remain): s runs through the
@@ -527,7 +539,7 @@ __device__ __forceinline__ void neighbor_list(NbCells c, const float4 p, const f
         const float ex = o.x - r.x, ey = o.y - r.y, ez = o.z - r.z;
         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
         const bool acc = (j != id) & (d2 < cell2) & (e2 > diam2) & (k < K);
-        if (acc) nb[koff] = j;
+        if (acc) nb.put(koff, j);
         koff += !acc ? 0u : !CHUNK ? n : (k & 3u) == 3u ? 4 * n - 3u : 1u;
         k += acc ? 1u : 0u;
     };
@@ -551,14 +563,14 @@ __device__ __forceinline__ void neighbor_list(NbCells c, const float4 p, const f
         }
         take(qb, rb);
     }
-    if (k < K) nb[koff] = VTX_INVALID;                           // a full list has no sentinel
+    if (k < K) nb.put(koff, VTX_INVALID);                        // a full list has no sentinel
 }
 
 // The whole 27-cell walk of one particle, its crange column the thread's own.
 template <int BS, bool CHUNK>
 __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, const uint32_t id,
                                               const float4* __restrict__ sP, const uint2* __restrict__ cells,
-                                              uint32_t* __restrict__ nb, const uint32_t n, const float cell,
+                                              const NbOut nb, const uint32_t n, const float cell,
                                               const float cell2, const float diam2, const uint32_t M,
                                               const uint64_t mc, const uint32_t K, const uint32_t shift,
                                               uint32_t (&crange)[VTX_NB_CACHE][BS]) {
@@ -608,11 +620,13 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     const uint64_t mc = seg_mod(sg);
     const float4* __restrict__ sP = d.sPX + 2 * (size_t)sg.pbase;   // position of sorted i at 2i, initial at 2i+1
     const uint2* __restrict__ cells = d.cells + sg.tbase;
-    uint32_t* __restrict__ table = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo);
+    char* const table = reinterpret_cast<char*>(d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo));
+    const bool n16 = sg.flags & VTX_SEG_NBR16;
+    const uint32_t esh = n16 ? 1u : 2u;   // log2 of the entry size
     if constexpr (!REDEAL) {
         const float4 p = d.P[in][sg.pbase + i], o = d.X0[sg.pbase + i];
-        neighbor_walk<BS, true>(p, o, i, sP, cells, table + 4 * (size_t)i, n, sg.cell, sg.cell2, sg.diam2, M, mc,
-                                K, shift, crange);
+        neighbor_walk<BS, true>(p, o, i, sP, cells, NbOut{table + ((size_t)4 * i << esh), n16}, n, sg.cell,
+                                sg.cell2, sg.diam2, M, mc, K, shift, crange);
     } else {
         NbCells c{0u, 0u};
         if (live) c = neighbor_cells<BS>(sP[2 * i], cells, sg.cell, M, mc, K, shift, crange, threadIdx.x);
@@ -639,8 +653,9 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
         const uint32_t tc = btotal[t];
         const float4 p = sP[2 * i], o = sP[2 * i + 1];
         const uint32_t id = __float_as_uint(p.w);
-        neighbor_list<BS, true>(NbCells{bmask[t], tc & 0xFFFFu}, p, o, id, sP, cells, table + 4 * (size_t)id, n,
-                                sg.cell, sg.cell2, sg.diam2, M, mc, K, shift, crange, tc >> 16);
+        neighbor_list<BS, true>(NbCells{bmask[t], tc & 0xFFFFu}, p, o, id, sP, cells,
+                                NbOut{table + ((size_t)4 * id << esh), n16}, n, sg.cell, sg.cell2, sg.diam2, M, mc,
+                                K, shift, crange, tc >> 16);
     }
 }
 
@@ -717,7 +732,8 @@ __global__ __launch_bounds__(BS) void k_raw_neighbors(const float4* __restrict__
     if (i >= n) return;
     const float4 p = rec[2 * (size_t)i], o = rec[2 * (size_t)i + 1];
     const uint32_t id = __float_as_uint(p.w);
-    neighbor_walk<BS, false>(p, o, id, rec, cells, nbr + id, n, cell, cell2, diam2, M, mc, K, 0u, crange);
+    neighbor_walk<BS, false>(p, o, id, rec, cells, NbOut{reinterpret_cast<char*>(nbr + id), false}, n, cell, cell2,
+                             diam2, M, mc, K, 0u, crange);
 }
 
 // The interleaved table to the caller's cell_start / cell_end (empty cells: ~0u in both).

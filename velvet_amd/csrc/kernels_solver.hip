@@ -163,9 +163,12 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
     const V3 x = xyz(d.X[g]);
     if (sg.flags & VTX_SEG_SELFCOL) {
         const uint32_t n = sg.n;
-        // the list in 16-byte chunks: entries 4c..4c+3 at chunk c, nb4[c * n]
-        const uint4* __restrict__ nb4 =
-            reinterpret_cast<const uint4*>(d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo)) + (g - sg.pbase);
+        // the list in chunks of four entries: chunk c at nb4[c * n] (32-bit entries) or nb2[c * n]
+        // (16-bit entries, 0xffff >= n the sentinel)
+        const uint32_t* __restrict__ table = d.nbr + ((uint64_t)sg.nbase_hi << 32 | sg.nbase_lo);
+        const uint4* __restrict__ nb4 = reinterpret_cast<const uint4*>(table) + (g - sg.pbase);
+        const uint2* __restrict__ nb2 = reinterpret_cast<const uint2*>(table) + (g - sg.pbase);
+        const bool n16 = sg.flags & VTX_SEG_NBR16;
         V3 delta = v3(0.f, 0.f, 0.f);
         int cnt = 0;
         const V3 vel_i = p - x;
@@ -179,7 +182,13 @@ __global__ __launch_bounds__(BS) void k_collide(DevState d, int in, float h) {
         const uint32_t K = sg.max_nbr;
         uint32_t jn[4];
         auto chunk = [&](uint32_t k0) {   // entries k0..k0+3, those at K or beyond as sentinels
-            const uint4 c = nb4[(size_t)(k0 >> 2) * n];
+            uint4 c;
+            if (n16) {
+                const uint2 h = nb2[(size_t)(k0 >> 2) * n];
+                c = make_uint4(h.x & 0xFFFFu, h.x >> 16, h.y & 0xFFFFu, h.y >> 16);
+            } else {
+                c = nb4[(size_t)(k0 >> 2) * n];
+            }
             jn[0] = c.x;
             jn[1] = k0 + 1 < K ? c.y : VTX_INVALID;
             jn[2] = k0 + 2 < K ? c.z : VTX_INVALID;

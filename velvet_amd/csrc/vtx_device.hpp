@@ -47,6 +47,12 @@
 // mirror image; shifted coordinates have no such partner nearby.
 #define VTX_SEG_HASH_SHIFT 2048u
 #define VTX_HASH_SHIFT 1000003u
+// The segment's neighbour table holds 16-bit entries (0xffff the sentinel): self-collision
+// segments of at most VTX_NBR16_N particles, whose indices and sentinel fit in 16 bits. Chunks
+// of four entries are then 8 bytes: half the table traffic of the search's stores and of the
+// contact pass's loads.
+#define VTX_SEG_NBR16 4096u
+#define VTX_NBR16_N 65535u
 // A particle's attachment terms are summed in four parts by slot index -- slots
 // [q*Qs, (q+1)*Qs) with Qs = ceil(scount / 4) -- each part from zero in slot order, and the parts
 // are added to the particle's delta in order q = 0..3. One lane can run all four parts, or four
@@ -144,7 +150,7 @@ struct DevState {
     uint2* cells;             // per hash cell: (start, end) in sorted order, start ~0u = empty
     float4* sPX;              // per sorted position, 32 B: the predicted position with w = local
                               // particle index, then the initial position (one cache line holds both)
-    uint32_t* nbr;            // neighbour tables per segment, 16-byte chunks of four entries
+    uint32_t* nbr;            // neighbour tables per segment, chunks of four entries (16 or 8 bytes)
     const TileDev* itiles;    // particle tiles of segments on the general (ELL) Jacobi path
     const GridTileDev* gtiles;
     const GridTileDev* ltiles;   // grid tiles whose dense attachments run on four lanes per particle
