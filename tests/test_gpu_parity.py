@@ -8,6 +8,7 @@
 * packing invariance: a segment gives bit-identical results alone or packed with others,
   graph or no graph, run after run.
 """
+import dataclasses
 import sys
 
 import numpy as np
@@ -89,6 +90,32 @@ def test_hash_stage_bit_exact_packed_segments():
         ref = oracle.hash32(p, c.x, cell, float(c.diameter), Mx)
         _check_hash_equal(s.read_hash(k), ref, c.n)
     s.close()
+
+
+def test_hash_nbr_entry_width_boundary():
+    """Neighbour tables at the 16-bit entry limit: a 65,535-particle segment (the largest with 16-bit
+    entries: indices up to 65,534, the sentinel 0xffff equal to n) packed with a 65,536-particle one
+    (32-bit entries); both read back bit-exact against the oracle, in every thread order."""
+    say("hash entry-width boundary n=65535 / 65536")
+    c, p = _folded(256, seed=11)
+    keep = 65535
+    tris = c.tris[np.all(c.tris < keep, axis=1)]
+    small = (dataclasses.replace(c, x=c.x[:keep].copy(), tris=tris), p[:keep].copy())
+    items = [small, (c, p)]
+    s = _hash_solver(items)
+    try:
+        for order in (1, 2, 0):
+            s.set_tuning(neighbor_order=order)
+            for k, (ck, pk) in enumerate(items):
+                s.write_predicted(pk, seg=k)
+            s.run_hash()
+            for k, (ck, pk) in enumerate(items):
+                n = ck.x.shape[0]
+                cell = float(F32(F32(ck.diameter) * F32(1.5)))
+                ref = oracle.hash32(pk, ck.x, cell, float(ck.diameter), 2 * n)
+                _check_hash_equal(s.read_hash(k), ref, n)
+    finally:
+        s.close()
 
 
 @pytest.mark.parametrize("hash_lds,order", [(1, 2), (0, 2), (1, 1), (0, 1), (1, 0)])
