@@ -471,12 +471,12 @@ __device__ __forceinline__ NbCells neighbor_cells(const float4 p, const uint2* _
 // The candidate walk after the sweep c (same function, same particle p / o / id; crange column col
 // from the sweep): sP the 32-B sorted records (position with w = index at 2i, initial position at
 // 2i + 1), cells the (start, end) table of M cells, shift the grid shift of the cell coordinates.
-// The list goes to nb, the particle's column of an n-particle table:
-//   CHUNK = false: entry k at nb[k * n] (column-major, the kernel-launch tier's layout);
-//   CHUNK = true:  entries in chunks of four, chunk c = entries 4c..4c+3 at nb[4 (c * n)]
-//                  (nb = table + 4 id, chunk-aligned): a lane's four consecutive entries share
-//                  one 16-byte (T = uint32_t) or 8-byte (T = uint16_t) run, and a reader loads
-//                  four at once.
+// The list goes to nb, the particle's column of an n-particle table (entry offsets):
+//   CHUNK = false: entry k at offset k * n (column-major, the kernel-launch tier's layout);
+//   CHUNK = true:  entries in chunks of four, chunk c = entries 4c..4c+3 at offset 4 (c * n)
+//                  (nb at the table's entry 4 id, chunk-aligned): a lane's four consecutive
+//                  entries share one 16-byte (32-bit entries) or 8-byte (16-bit entries) run, and
+//                  a reader loads four at once.
 // Entries are 32-bit or, for a segment with VTX_SEG_NBR16, 16-bit (the sentinel truncated to
 // 0xffff); nb.put(k, v) stores entry offset k.
 struct NbOut {
