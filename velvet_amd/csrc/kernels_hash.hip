@@ -266,7 +266,10 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
 // three workgroup barriers per pass), the cell table (k_clear_cells + k_cell_bounds) and the
 // sorted records. The sort runs on one CU and is bound by its VALU issue (a wave64 instruction
 // takes four cycles of a 16-lane SIMD), hence the atomic counts: the ballot match is needed only
-// for the ranks of the stable scatter.
+// for the ranks of the stable scatter. The records are written from particle order through the
+// inverse permutation (kept in the free sort buffer): the positions and initial positions are
+// read coalesced and each 32-B record is one scattered write, where the sorted-order form gathers
+// two scattered 16-B reads per particle (batch of 512 cloths: 99.6 -> 79.2 us per build).
 // Same keys, same stable order, so the same table and records as the device-wide path, in one
 // launch for every small segment instead of a dozen over all of them.
 // NMAX: particles per segment at most (16384: 144 KB of LDS, one workgroup per CU). Global
@@ -362,38 +365,42 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
         __syncthreads();
         cur ^= 1;
     }
-    // sorted keys and values, cell start / end, sorted records (as k_cell_bounds)
+    // sorted keys and values, cell start / end (as k_cell_bounds); the inverse permutation
     uint32_t* __restrict__ ko = d.keys[dst] + sg.pbase;
     uint32_t* __restrict__ vo = d.vals[dst] + sg.pbase;
-    for (uint32_t g0 = tid; g0 < n; g0 += STEP) {
+    uint32_t* __restrict__ inv = buf[cur ^ 1];
+    for (uint32_t g = tid; g < n; g += VTX_HASH_LDS_BT) {
+        const uint32_t e = buf[cur][g], key = e >> 16, id = e & 0xFFFFu;
+        inv[id] = g;
+        ko[g] = key;
+        vo[g] = id;
+        if (g == 0) {
+            cells[key].x = 0;
+        } else {
+            const uint32_t prev = buf[cur][g - 1] >> 16;
+            if (key != prev) {
+                cells[key].x = g;
+                cells[prev].y = g;
+            }
+        }
+        if (g == n - 1) cells[key].y = n;
+    }
+    __syncthreads();
+    // sorted records written from particle order: coalesced reads, scattered 32-B records
+    for (uint32_t i0 = tid; i0 < n; i0 += STEP) {
         float4 q[U], x0[U];
-        uint32_t ids[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t g = g0 + u * VTX_HASH_LDS_BT;
-            ids[u] = g < n ? buf[cur][g] & 0xFFFFu : 0u;
-            q[u] = g < n ? P[ids[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
-            x0[u] = g < n ? d.X0[sg.pbase + ids[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
+            q[u] = i < n ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            x0[u] = i < n ? d.X0[sg.pbase + i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t g = g0 + u * VTX_HASH_LDS_BT;
-            if (g >= n) continue;
-            const uint32_t key = buf[cur][g] >> 16, id = ids[u];
-            ko[g] = key;
-            vo[g] = id;
-            if (g == 0) {
-                cells[key].x = 0;
-            } else {
-                const uint32_t prev = buf[cur][g - 1] >> 16;
-                if (key != prev) {
-                    cells[key].x = g;
-                    cells[prev].y = g;
-                }
-            }
-            if (g == n - 1) cells[key].y = n;
-            const size_t o = 2 * (size_t)(sg.pbase + g);
-            d.sPX[o] = make_float4(q[u].x, q[u].y, q[u].z, __uint_as_float(id));
+            const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
+            if (i >= n) continue;
+            const size_t o = 2 * (size_t)(sg.pbase + inv[i]);
+            d.sPX[o] = make_float4(q[u].x, q[u].y, q[u].z, __uint_as_float(i));
             d.sPX[o + 1] = x0[u];
         }
     }
