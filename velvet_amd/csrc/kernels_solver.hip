@@ -129,7 +129,10 @@ __device__ V3 collide_sdf(V3 pred, V3 pos, const ColliderDev* cols, uint32_t cb,
 }
 
 // Pre-stabilisation SDF on positions with the frame time (Velvet/VtClothSolverGPU.hpp:73) and
-// the first substep's PredictPositions (cu:42-53).
+// the first substep's PredictPositions (cu:42-53). The predicted velocity is not stored: every
+// substep ends in Finalize, which derives the velocity from the positions alone and stores it, so
+// between frames (where the host and the failure check read it) V holds Finalize's value, as in
+// the reference, and nothing reads V inside a frame.
 __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float frame_dt, float h) {
     const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
@@ -141,8 +144,7 @@ __global__ __launch_bounds__(VTX_TILE) void k_frame_begin(DevState d, float fram
         x = collide_sdf(x, x, d.cols, sg.cbase, sg.ccount, sg.margin, sg.friction, frame_dt);
         d.X[g] = f4(x, x4.w);
     }
-    V3 v = xyz(d.V[g]) + v3(sg.gx, sg.gy, sg.gz) * h;
-    d.V[g] = f4(v, 0.f);
+    const V3 v = xyz(d.V[g]) + v3(sg.gx, sg.gy, sg.gz) * h;
     d.P[0][g] = f4(x + v * h, x4.w);
 }
 
@@ -403,7 +405,8 @@ __device__ __forceinline__ bool bend_eval(float4 q0, float4 q1, float4 q2, float
     return true;
 }
 
-// Finalize (Velvet/VtClothSolverGPU.cu:396-406) and optionally the next substep's Predict.
+// Finalize (Velvet/VtClothSolverGPU.cu:396-406) and optionally the next substep's Predict, whose
+// velocity is not stored (the next Finalize overwrites it; see k_frame_begin).
 template <int EPI>
 __device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, uint32_t g, int in, V3 p, float w,
                                          float h) {
@@ -423,8 +426,9 @@ __device__ __forceinline__ void epilogue(const DevState& d, const SegDev& sg, ui
         if (EPI == EPI_FINALIZE_PREDICT) {
             v = v + v3(sg.gx, sg.gy, sg.gz) * h;
             d.P[in ^ 1][g] = f4(p + v * h, x4.w);
+        } else {
+            d.V[g] = f4(v, 0.f);
         }
-        d.V[g] = f4(v, 0.f);
     }
 }
 
