@@ -1489,7 +1489,7 @@ void Engine::hash_build(const DevState& d, int in, hipStream_t s) {
             src ^= 1;
         }
         stage_end(s);
-        stage_begin("k_clear_cells+k_cell_bounds", "Solver_HashBuildCell", s);
+        stage_begin("k_clear_cells+k_cell_bounds+k_sorted_records", "Solver_HashBuildCell", s);
         VTX_HIP(launch_clear_cells(d, s));
         VTX_HIP(launch_cell_bounds(d, in, src, s));
         stage_end(s);

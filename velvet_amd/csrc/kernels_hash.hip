@@ -229,8 +229,9 @@ __global__ __launch_bounds__(VTX_TILE) void k_clear_cells(DevState d) {
 }
 
 // FindCellStart_Kernel (Velvet/SpatialHashGPU.cu:44-77) per segment with local sorted
-// positions (cells were filled with ~0u: empty), plus the gather of predicted and initial
-// positions into sorted order for the neighbour search.
+// positions (cells were filled with ~0u: empty), plus the inverse permutation (particle ->
+// sorted position) in the sort's free key buffer, from which k_sorted_records writes the
+// sorted records in particle order.
 __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, int src) {
     const TileDev t = d.tiles[tile_of(d.remap)];
     if (threadIdx.x >= t.count) return;
@@ -251,12 +252,24 @@ __global__ __launch_bounds__(VTX_TILE) void k_cell_bounds(DevState d, int in, in
         }
     }
     if (local == sg.n - 1) cells[key].y = sg.n;
-    // sorted copies for the neighbour search; w of the position copy carries the particle's
-    // local index (the inverse mass is not needed there)
-    const uint32_t id = d.vals[src][g];
-    const float4 q = d.P[in][sg.pbase + id];
-    d.sPX[2 * (size_t)g] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
-    d.sPX[2 * (size_t)g + 1] = d.X0[sg.pbase + id];
+    d.keys[src ^ 1][sg.pbase + d.vals[src][g]] = local;
+}
+
+// The sorted copies for the neighbour search, written from particle order: coalesced reads of the
+// predicted and initial positions, one scattered 32-B record per particle (a gather in sorted
+// order reads two scattered 16-B halves instead). w of the position copy carries the particle's
+// local index (the inverse mass is not needed there). inv = the inverse permutation k_cell_bounds
+// left in the sort's free key buffer.
+__global__ __launch_bounds__(VTX_TILE) void k_sorted_records(DevState d, int in, int src) {
+    const TileDev t = d.tiles[tile_of(d.remap)];
+    if (threadIdx.x >= t.count) return;
+    const SegDev sg = d.segs[t.seg];
+    if (!(sg.flags & VTX_SEG_SELFCOL) || (sg.flags & VTX_SEG_HASH_LDS)) return;
+    const uint32_t g = t.begin + threadIdx.x;
+    const size_t o = 2 * (size_t)(sg.pbase + d.keys[src ^ 1][g]);
+    const float4 q = d.P[in][g];
+    d.sPX[o] = make_float4(q.x, q.y, q.z, __uint_as_float(g - sg.pbase));
+    d.sPX[o + 1] = d.X0[g];
 }
 
 // The whole spatial-hash build of one small segment in one workgroup of VTX_HASH_LDS_BT threads:
@@ -899,6 +912,7 @@ hipError_t launch_clear_cells(const DevState& d, hipStream_t s) {
 
 hipError_t launch_cell_bounds(const DevState& d, int in, int src, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_bounds, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
+    hipLaunchKernelGGL(k_sorted_records, dim3(d.ntiles), dim3(VTX_TILE), 0, s, d, in, src);
     return hipGetLastError();
 }
 
