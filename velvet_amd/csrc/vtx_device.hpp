@@ -16,7 +16,7 @@
 #define VTX_TILE 256
 #define VTX_SORT_ITEMS 8                       // keys per thread in a sort tile
 #define VTX_SORT_TILE (VTX_TILE * VTX_SORT_ITEMS)
-#define VTX_SCAN_BLOCK 4096                     // elements per block of the device scan
+#define VTX_SCAN_BLOCK 1024                     // elements per block of the device scan
 #define VTX_INVALID 0xFFFFFFFFu
 #define VTX_EPSILON 1e-6f                       // Velvet/Common.cuh:21
 #define VTX_MAX_NEIGHBORS 64
