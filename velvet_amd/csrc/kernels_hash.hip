@@ -156,54 +156,60 @@ __global__ __launch_bounds__(VTX_TILE) void k_scan_apply(uint32_t* __restrict__ 
     }
 }
 
-// Stable scatter of one sort tile (count keys and values from kin / vin): rounds of 256 keys in
-// order; inside a round, ranks come from a wave match on the 8 digit bits (ballots) and the
-// waves' per-digit counts. Digit g of the tile starts at output position scanned[g * stride] + bias.
+// Stable scatter of one sort tile (count keys and values from kin / vin). Wave w owns the tile's
+// contiguous chunk w of count / 4 keys: it counts its chunk's digits with LDS atomics (order does
+// not matter), each digit's start then runs through the waves in order from the tile's base
+// (digit g of the tile starts at output position scanned[g * stride] + bias), and each wave
+// scatters its chunk in order, 64 keys at a time ranked by ballots over the 8 digit bits, moving
+// its own per-digit offsets on: chunks in wave order and keys in chunk order keep the sort
+// stable, with two workgroup barriers per tile.
 __device__ __forceinline__ void tile_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                              uint32_t count, int shift, const uint32_t* __restrict__ scanned,
                                              uint32_t stride, uint32_t bias, uint32_t* __restrict__ kout,
                                              uint32_t* __restrict__ vout) {
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t wcnt[VTX_TILE / 64][256];
+    constexpr uint32_t NW = VTX_TILE / 64;
+    __shared__ uint32_t wcnt[NW][256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    base[tid] = scanned[tid * stride];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t r = 0; r < VTX_SORT_ITEMS; r++) {
-        const uint32_t i = r * VTX_TILE + tid;
-        const bool valid = i < count;
+    const uint32_t chunk = (count + NW - 1) / NW;
+    const uint32_t c0 = min(wave * chunk, count), c1 = min(c0 + chunk, count);
+    uint32_t* __restrict__ hw = wcnt[wave];
+    for (uint32_t k = lane; k < 256; k += 64) hw[k] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = c0 + lane; i < c1; i += 64) atomicAdd(&hw[(kin[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    {   // thread tid = digit: the waves' starts in wave order
+        uint32_t run = scanned[tid * stride] + bias;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; w++) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t r0 = c0; r0 < c1; r0 += 64) {
+        const uint32_t i = r0 + lane;
+        const bool valid = i < c1;
         const uint32_t key = valid ? kin[i] : 0u;
         const uint32_t val = valid ? vin[i] : 0u;
         const uint32_t dg = (key >> shift) & 255u;
-#pragma unroll
-        for (uint32_t w = 0; w < VTX_TILE / 64; w++) wcnt[w][tid] = 0;
-        __syncthreads();
-        uint64_t peers = __ballot(valid);
+        uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const bool bit = (dg >> b) & 1u;
-            const uint64_t bal = __ballot(valid && bit);
-            peers &= bit ? bal : ~bal;
+            const uint32_t m = (uint32_t)((int32_t)(dg << (31 - b)) >> 31);   // bit b as 0 / ~0
+            const uint64_t bal = __builtin_amdgcn_ballot_w64(m != 0u);
+            peers &= ~(bal ^ ((uint64_t)m << 32 | m));                        // lanes whose bit b is this lane's
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t run = base[tid];
-#pragma unroll
-            for (uint32_t w = 0; w < VTX_TILE / 64; w++) {
-                const uint32_t c = wcnt[w][tid];
-                wcnt[w][tid] = run;
-                run += c;
-            }
-            base[tid] = run;
-        }
-        __syncthreads();
+        const uint32_t at = hw[dg];
         if (valid) {
-            const uint32_t dst = wcnt[wave][dg] + rank + bias;
-            kout[dst] = key;
-            vout[dst] = val;
+            kout[at + rank] = key;
+            vout[at + rank] = val;
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) hw[dg] = at + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
