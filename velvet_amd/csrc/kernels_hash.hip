@@ -6,8 +6,9 @@
 // digit histograms are laid out (segment, digit, tile), so one exclusive scan over the whole
 // histogram gives every (segment, digit, tile) its global output offset: the segments come
 // out sorted independently, in place, without sorting on segment bits. Ranking inside a tile
-// is stable (wave ballots, then waves in order, then rounds in order), so the result equals a
-// stable sort by hash within each segment -- the contract of the reference's
+// is stable (each wave a contiguous chunk, in wave order; inside a chunk 64 keys at a time by
+// wave ballots, in order), so the result equals a stable sort by hash within each segment -- the
+// contract of the reference's
 // cub::DeviceRadixSort::SortPairs call (Velvet/SpatialHashGPU.cu:133-157).
 #include <hip/hip_runtime.h>
 
