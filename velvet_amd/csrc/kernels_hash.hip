@@ -309,8 +309,15 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
     const uint64_t mc = seg_mod(sg);
     const uint32_t sh = seg_shift(sg);
     const float4* __restrict__ P = d.P[in] + sg.pbase;
-    constexpr uint32_t U = 8, STEP = U * VTX_HASH_LDS_BT;
-    for (uint32_t i0 = tid; i0 < n; i0 += STEP) {
+    // the predicted positions stay in registers (NMAX / STEP rounds of U per thread) until the
+    // sorted records are written: the workgroup runs alone on its CU, so the 48 extra VGPRs cost
+    // no occupancy, and the records then read only the initial positions (75.0 against 78.7 us for
+    // the batch of 512 cloths)
+    constexpr uint32_t U = 8, STEP = U * VTX_HASH_LDS_BT, NS = (NMAX + STEP - 1) / STEP;
+    float kx[NS][U], ky[NS][U], kz[NS][U];
+#pragma unroll
+    for (uint32_t s0 = 0; s0 < NS; s0++) {
+        const uint32_t i0 = tid + s0 * STEP;
         float4 p[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
@@ -320,6 +327,9 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
+            kx[s0][u] = p[u].x;
+            ky[s0][u] = p[u].y;
+            kz[s0][u] = p[u].z;
             if (i < n)
                 buf[0][i] = hash_coords(hash_coord(p[u].x, sg.cell, sh), hash_coord(p[u].y, sg.cell, sh),
                                         hash_coord(p[u].z, sg.cell, sh), M, mc) << 16 | i;
@@ -407,12 +417,13 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
     }
     __syncthreads();
     // sorted records written from particle order: coalesced reads, scattered 32-B records
-    for (uint32_t i0 = tid; i0 < n; i0 += STEP) {
-        float4 q[U], x0[U];
+#pragma unroll
+    for (uint32_t s0 = 0; s0 < NS; s0++) {
+        const uint32_t i0 = tid + s0 * STEP;
+        float4 x0[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
-            q[u] = i < n ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
             x0[u] = i < n ? d.X0[sg.pbase + i] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
@@ -420,7 +431,7 @@ __global__ __launch_bounds__(VTX_HASH_LDS_BT) void k_hash_small(DevState d, cons
             const uint32_t i = i0 + u * VTX_HASH_LDS_BT;
             if (i >= n) continue;
             const size_t o = 2 * (size_t)(sg.pbase + inv[i]);
-            d.sPX[o] = make_float4(q[u].x, q[u].y, q[u].z, __uint_as_float(i));
+            d.sPX[o] = make_float4(kx[s0][u], ky[s0][u], kz[s0][u], __uint_as_float(i));
             d.sPX[o + 1] = x0[u];
         }
     }
