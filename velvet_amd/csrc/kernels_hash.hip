@@ -618,7 +618,7 @@ __device__ __forceinline__ void neighbor_walk(const float4 p, const float4 o, co
 }
 
 // In hash-sorted order (NBR_SORTED) the workgroup's threads take its particles again after the
-// sweep, in descending order of their candidate totals (buckets of 8 candidates, ties in any
+// sweep, in descending order of their candidate totals (buckets of 4 candidates, ties in any
 // order): the walk of a wave lasts as long as its longest list, and lanes of similar totals
 // (among them the lanes of one cell) waste fewer trips. A thread reads its new particle's sweep
 // from the LDS column of the thread that swept it. The lists do not depend on which thread walks
@@ -668,12 +668,12 @@ __global__ __launch_bounds__(BS) void k_neighbors(DevState d, const TileDev* __r
     } else {
         NbCells c{0u, 0u};
         if (live) c = neighbor_cells<BS>(sP[2 * i], cells, sg.cell, M, mc, K, shift, crange, threadIdx.x);
-        // descending totals: bucket VTX_NB_BUCKETS - 1 - total / 8, idle threads last
+        // descending totals: bucket VTX_NB_BUCKETS - 1 - total / 4, idle threads last
         const uint32_t t = threadIdx.x;
         if (t < VTX_NB_BUCKETS) bcount[t] = 0;
         __syncthreads();
         const uint32_t key =
-            live ? VTX_NB_BUCKETS - 1u - min(c.total >> 3, VTX_NB_BUCKETS - 1u) : VTX_NB_BUCKETS - 1u;
+            live ? VTX_NB_BUCKETS - 1u - min(c.total >> 2, VTX_NB_BUCKETS - 1u) : VTX_NB_BUCKETS - 1u;
         const uint32_t slot = atomicAdd(&bcount[key], 1u);
         __syncthreads();
         if (t < VTX_NB_BUCKETS) {
